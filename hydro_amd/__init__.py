@@ -1,0 +1,229 @@
+"""hydro_amd -- MI355X-native HydroC99 split Godunov/MUSCL step.
+
+Python mirror of the C ABI in ``include/hydro_amd.h`` (ctypes over the in-tree
+``hydro_amd/libhydro_amd.so``).  All arithmetic runs in the HIP kernels of that
+library; this module only moves parameters and host arrays.  If the library is
+missing, importing the bindings raises -- there is no CPU fallback.
+
+Arrays use the reference layout: ``uold[4, ny+4, nx+4]`` float64 (slot 0 rho,
+1 rho*u, 2 rho*v, 3 E, two ghost layers), as in
+HydroC99_2DMpi/Src/parametres.h:102-107.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+import subprocess
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libhydro_amd.so")
+
+FP64 = 0
+MIXED = 1
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def build(jobs: int = 8) -> None:
+    """Compile libhydro_amd.so and bin/hydro in place (hipcc, gfx950)."""
+    subprocess.run(["make", "-s", "-C", PKG_DIR, f"-j{jobs}"], check=True)
+
+
+class HydroError(RuntimeError):
+    pass
+
+
+class _CParams(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("gamma", C.c_double),
+                ("courant_factor", C.c_double), ("smallc", C.c_double), ("smallr", C.c_double),
+                ("slope_type", C.c_double), ("niter_riemann", C.c_int), ("iorder", C.c_int),
+                ("scheme", C.c_int), ("boundary_left", C.c_int), ("boundary_right", C.c_int),
+                ("boundary_down", C.c_int), ("boundary_up", C.c_int), ("tend", C.c_double),
+                ("nstepmax", C.c_int), ("precision", C.c_int)]
+
+
+@dataclasses.dataclass
+class Params:
+    """hydro_amd_params; defaults are HydroC99_2DMpi/Src/parametres.c:64-109."""
+    nx: int = 20
+    ny: int = 20
+    dx: float = 1.0
+    gamma: float = 1.4
+    courant_factor: float = 0.5
+    smallc: float = 1e-10
+    smallr: float = 1e-10
+    slope_type: float = 1.0
+    niter_riemann: int = 10
+    iorder: int = 2
+    scheme: int = 1
+    boundary_left: int = 1
+    boundary_right: int = 1
+    boundary_down: int = 1
+    boundary_up: int = 1
+    tend: float = 0.0
+    nstepmax: int = 2**31 - 1
+    precision: int = FP64
+
+    def c(self) -> _CParams:
+        return _CParams(*[getattr(self, f[0]) for f in _CParams._fields_])
+
+    def shape(self):
+        return (4, self.ny + 4, self.nx + 4)
+
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    """The loaded native library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise HydroError(f"{LIB_PATH} is missing: build it with hydro_amd.build() "
+                         "(make -C hydro_amd); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    G = C.c_void_p
+    P = C.POINTER(_CParams)
+    L.hydro_amd_last_error.restype = C.c_char_p
+    L.hydro_amd_grid_create.argtypes = [P]
+    L.hydro_amd_grid_create.restype = G
+    L.hydro_amd_grid_destroy.argtypes = [G]
+    L.hydro_amd_grid_destroy.restype = None
+    for name, args in [("hydro_amd_grid_init", [G, C.c_int]),
+                       ("hydro_amd_grid_init_blast", [G, C.c_int, C.c_int, C.c_double]),
+                       ("hydro_amd_grid_upload", [G, _dp]),
+                       ("hydro_amd_grid_download", [G, _dp]),
+                       ("hydro_amd_grid_godunov", [G, C.c_int, C.c_double]),
+                       ("hydro_amd_grid_make_boundary", [G, C.c_int]),
+                       ("hydro_amd_grid_compute_deltat", [G, C.POINTER(C.c_double)]),
+                       ("hydro_amd_grid_run", [G, C.c_int, C.c_void_p]),
+                       ("hydro_amd_grid_run_async", [G, C.c_int]),
+                       ("hydro_amd_grid_sync", [G]),
+                       ("hydro_amd_grid_state", [G, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                                 C.POINTER(C.c_double)]),
+                       ("hydro_amd_grid_set_state", [G, C.c_double, C.c_int, C.c_double]),
+                       ("hydro_amd_grid_set_limits", [G, C.c_double, C.c_int]),
+                       ("hydro_amd_grid_set_tune", [G, C.c_int, C.c_int]),
+                       ("hydro_amd_grid_set_profile", [G, C.c_int]),
+                       ("hydro_amd_grid_profile", [G, _dp, C.POINTER(C.c_longlong)])]:
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.hydro_amd_grid_bytes.argtypes = [G]
+    L.hydro_amd_grid_bytes.restype = C.c_longlong
+    _LIB = L
+    return L
+
+
+def last_error() -> str:
+    return lib().hydro_amd_last_error().decode()
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise HydroError(last_error())
+    return rc
+
+
+class Grid:
+    """One grid on the GPU (hydro_amd_grid_*)."""
+
+    def __init__(self, p: Params):
+        self.p = p
+        self._h = lib().hydro_amd_grid_create(C.byref(p.c()))
+        if not self._h:
+            raise HydroError(last_error())
+
+    def close(self) -> None:
+        if self._h:
+            lib().hydro_amd_grid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # state
+    def init(self, testcase: int) -> None:
+        _check(lib().hydro_amd_grid_init(self._h, testcase))
+
+    def init_blast(self, bx: int, by: int, e: float) -> None:
+        _check(lib().hydro_amd_grid_init_blast(self._h, bx, by, e))
+
+    def upload(self, u: np.ndarray) -> None:
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.shape != self.p.shape():
+            raise ValueError(f"uold shape {u.shape} != {self.p.shape()}")
+        _check(lib().hydro_amd_grid_upload(self._h, u))
+
+    def download(self) -> np.ndarray:
+        u = np.empty(self.p.shape())
+        _check(lib().hydro_amd_grid_download(self._h, u))
+        return u
+
+    # the reference call surface on this grid
+    def godunov(self, idim: int, dt: float) -> None:
+        _check(lib().hydro_amd_grid_godunov(self._h, idim, dt))
+
+    def make_boundary(self, idim: int) -> None:
+        _check(lib().hydro_amd_grid_make_boundary(self._h, idim))
+
+    def compute_deltat(self) -> float:
+        d = C.c_double(0.0)
+        _check(lib().hydro_amd_grid_compute_deltat(self._h, C.byref(d)))
+        return d.value
+
+    # device time loop
+    def run(self, max_steps: int, log: bool = True):
+        """Run up to max_steps steps on the GPU; returns the dt of each step."""
+        buf = np.zeros(max(max_steps, 1)) if log else None
+        ptr = buf.ctypes.data_as(C.c_void_p) if log else None
+        n = _check(lib().hydro_amd_grid_run(self._h, max_steps, ptr))
+        return buf[:n].copy() if log else n
+
+    def run_async(self, max_steps: int) -> None:
+        _check(lib().hydro_amd_grid_run_async(self._h, max_steps))
+
+    def sync(self) -> None:
+        _check(lib().hydro_amd_grid_sync(self._h))
+
+    def state(self):
+        t, n, dt = C.c_double(), C.c_int(), C.c_double()
+        _check(lib().hydro_amd_grid_state(self._h, C.byref(t), C.byref(n), C.byref(dt)))
+        return t.value, n.value, dt.value
+
+    def set_state(self, t: float, nstep: int, dt: float) -> None:
+        _check(lib().hydro_amd_grid_set_state(self._h, t, nstep, dt))
+
+    def set_limits(self, tend: float, nstepmax: int) -> None:
+        _check(lib().hydro_amd_grid_set_limits(self._h, tend, nstepmax))
+
+    def set_tune(self, yseg: int, ywaves: int) -> None:
+        _check(lib().hydro_amd_grid_set_tune(self._h, yseg, ywaves))
+
+    def set_profile(self, on: bool) -> None:
+        _check(lib().hydro_amd_grid_set_profile(self._h, int(on)))
+
+    def profile(self):
+        """{kernel: (total_ms, launches)} for x_sweep, y_sweep, boundary, courant."""
+        ms = np.zeros(4)
+        n = (C.c_longlong * 4)()
+        _check(lib().hydro_amd_grid_profile(self._h, ms, n))
+        return {k: (float(ms[i]), int(n[i]))
+                for i, k in enumerate(("x_sweep", "y_sweep", "boundary", "courant"))}
+
+    @property
+    def nbytes(self) -> int:
+        return int(lib().hydro_amd_grid_bytes(self._h))

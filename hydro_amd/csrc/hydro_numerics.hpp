@@ -1,0 +1,301 @@
+// Per-cell and per-face numerics of HydroC99's split Godunov/MUSCL step, as
+// inline device functions shared by every sweep kernel (single grids, batched
+// grids, FP64 and mixed precision).
+//
+// Parity rules (FP64 form, R = double): every expression keeps the reference's
+// evaluation order (C's left-to-right association), the library is compiled
+// with -ffp-contract=off so no a*b+c is fused, and '/' and sqrt are the
+// correctly rounded hardware sequences.  With those, each function returns the
+// same bits as the reference routine cited beside it.
+//
+// Mixed precision (R = float): storage, primitives, slopes, traces, wave
+// speeds and fluxes are float; the Riemann first guess and Newton pressure
+// iteration run in double (riemann_face), the update accumulates in double
+// (update_cell) and the CFL maximum is kept in double by the callers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace hydro {
+
+// Physical / scheme constants, precomputed per launch in the compute type.
+template <typename R>
+struct Phys {
+  R gamma, gm1, smallr, smallc, smallp, entho, slope_type;
+  double gamma_d, gamma6_d, smallp_d, smallpp_d;  // Newton constants (double)
+  int niter;      // niter_riemann
+  int iorder;     // 1: first order (dq = 0)
+  int scheme;     // 1 muscl, 2 plmde, 3 collela
+};
+
+template <typename R>
+__host__ __device__ inline Phys<R> make_phys(double gamma, double smallr, double smallc,
+                                             double slope_type, int niter, int iorder,
+                                             int scheme) {
+  Phys<R> p;
+  // equation_of_state.c:60 / riemann.c:97-99 / cmpflx.c:60
+  p.gamma = (R) gamma;
+  p.gm1 = (R) gamma - (R) 1;
+  p.smallr = (R) smallr;
+  p.smallc = (R) smallc;
+  p.smallp = ((R) smallc * (R) smallc) / (R) gamma;
+  p.entho = (R) 1 / ((R) gamma - (R) 1);
+  p.slope_type = (R) slope_type;
+  p.gamma_d = gamma;
+  p.gamma6_d = (gamma + 1.0) / (2.0 * gamma);
+  p.smallp_d = (smallc * smallc) / gamma;
+  p.smallpp_d = smallr * p.smallp_d;
+  p.niter = niter;
+  p.iorder = iorder;
+  p.scheme = scheme;
+  return p;
+}
+
+template <typename R>
+struct Q4 {  // rho, normal velocity, transverse velocity, pressure
+  R r, u, v, p;
+};
+
+template <typename R>
+struct Prim {
+  Q4<R> q;
+  R c;  // sound speed
+};
+
+template <typename R>
+__device__ __forceinline__ R maxm(R x, R y) { return x > y ? x : y; }  // utils.h:47 MAX
+
+// constoprim.c:62-78 + equation_of_state.c:64-81 for one cell.  U is in sweep
+// slot order (slot 1 = normal momentum).
+template <typename R>
+__device__ __forceinline__ Prim<R> prim_cell(R U0, R U1, R U2, R U3, const Phys<R>& ph) {
+  Prim<R> o;
+  const R half = (R) 0.5;
+  R r = maxm(U0, ph.smallr);
+  R u = U1 / r;
+  R v = U2 / r;
+  R eken = half * (u * u + v * v);
+  R e = U3 / r - eken;
+  R p = ph.gm1 * r * e;
+  p = maxm(p, r * ph.smallp);
+  o.c = sqrt(ph.gamma * p / r);
+  o.q.r = r;
+  o.q.u = u;
+  o.q.v = v;
+  o.q.p = p;
+  return o;
+}
+
+// Courant numbers of one cell in grid slot order (compute_deltat.c:71-88 +
+// courantOnXY.c:115-121): returns c + |vx| and c + |vy|.
+template <typename R>
+__device__ __forceinline__ void courant_cell(R U0, R U1, R U2, R U3, const Phys<R>& ph, R& cx,
+                                             R& cy) {
+  Prim<R> o = prim_cell(U0, U1, U2, U3, ph);
+  cx = o.c + fabs(o.q.u);
+  cy = o.c + fabs(o.q.v);
+}
+
+// slope.c:71-90 (non-OLDSTYLE limiter) for one variable.  ST1 specialises
+// slope_type == 1, where st*x == x and x/st == x exactly.
+template <typename R, bool ST1>
+__device__ __forceinline__ R slope1(R qm, R q0, R qp, R st) {
+  const R half = (R) 0.5;
+  R dl, dr, dc;
+  if (ST1) {
+    dl = q0 - qm;
+    dr = qp - q0;
+    dc = half * (dl + dr);
+  } else {
+    dl = st * (q0 - qm);
+    dr = st * (qp - q0);
+    dc = half * (dl + dr) / st;
+  }
+  R sg = (dc > (R) 0) ? (R) 1 : (R) -1;
+  R keep = ((dl * dr) <= (R) 0) ? (R) 0 : (R) 1;
+  R t1 = fmin(fabs(dl), fabs(dr));
+  return sg * fmin(keep * t1, fabs(dc));
+}
+
+template <typename R, bool ST1>
+__device__ __forceinline__ Q4<R> slope4(const Q4<R>& qm, const Q4<R>& q0, const Q4<R>& qp,
+                                        R st) {
+  Q4<R> d;
+  d.r = slope1<R, ST1>(qm.r, q0.r, qp.r, st);
+  d.u = slope1<R, ST1>(qm.u, q0.u, qp.u, st);
+  d.v = slope1<R, ST1>(qm.v, q0.v, qp.v, st);
+  d.p = slope1<R, ST1>(qm.p, q0.p, qp.p, st);
+  return d;
+}
+
+// Trace constants of trace.c:63-80.
+template <typename R>
+struct TraceK {
+  R dtdx, zerol, zeror, project;
+};
+
+template <typename R>
+__device__ __forceinline__ TraceK<R> make_tracek(R dtdx, int scheme) {
+  TraceK<R> k;
+  k.dtdx = dtdx;
+  k.zerol = (R) 0;
+  k.zeror = (R) 0;
+  k.project = (R) 0;
+  if (scheme == 1) {
+    k.zerol = -(R) 100 / dtdx;
+    k.zeror = (R) 100 / dtdx;
+    k.project = (R) 1;
+  } else if (scheme == 2) {
+    k.project = (R) 1;
+  }
+  return k;
+}
+
+// trace.c:85-147: MUSCL-Hancock predictor of one cell -> qxm (the state at
+// the cell's right face, seen from inside) and qxp (its left face).
+template <typename R>
+__device__ __forceinline__ void trace_cell(const Q4<R>& q, const Q4<R>& dq, R cc,
+                                           const TraceK<R>& k, Q4<R>& qxm, Q4<R>& qxp) {
+  const R half = (R) 0.5;
+  const R one = (R) 1;
+  R csq = cc * cc;
+  R rOcc = q.r / cc;
+  R OrOcc = cc / q.r;
+  R dprcc = dq.p / (q.r * cc);
+  R am = half * (dprcc - dq.u) * rOcc;
+  R ap = half * (dprcc + dq.u) * rOcc;
+  R a0r = dq.r - dq.p / csq;
+  R a0v = dq.v;
+  R upcc = q.u + cc;
+  R umcc = q.u - cc;
+  R upccx = upcc * k.dtdx;
+  R umccx = umcc * k.dtdx;
+  R ux = q.u * k.dtdx;
+  {
+    R spm = (umcc >= k.zeror) ? k.project : umccx + one;
+    R spp = (upcc >= k.zeror) ? k.project : upccx + one;
+    R sp0 = (q.u >= k.zeror) ? k.project : ux + one;
+    R apr = -half * spp * ap;
+    R amr = -half * spm * am;
+    R azr = -half * sp0 * a0r;
+    R azv = -half * sp0 * a0v;
+    qxp.r = q.r + (apr + amr + azr);
+    qxp.u = q.u + (apr - amr) * OrOcc;
+    qxp.v = q.v + (azv);
+    qxp.p = q.p + (apr + amr) * csq;
+  }
+  {
+    R spm = (umcc <= k.zerol) ? -k.project : umccx - one;
+    R spp = (upcc <= k.zerol) ? -k.project : upccx - one;
+    R sp0 = (q.u <= k.zerol) ? -k.project : ux - one;
+    R apl = -half * spp * ap;
+    R aml = -half * spm * am;
+    R azl = -half * sp0 * a0r;
+    R azv = -half * sp0 * a0v;
+    qxm.r = q.r + (apl + aml + azl);
+    qxm.u = q.u + (apl - aml) * OrOcc;
+    qxm.v = q.v + (azv);
+    qxm.p = q.p + (apl + aml) * csq;
+  }
+}
+
+// riemann.c:128-276 for one face (qleft = ql, qright = qr) followed by
+// cmpflx.c:67-90: returns the Godunov flux through the face.
+template <typename R>
+__device__ __forceinline__ Q4<R> riemann_flux(const Q4<R>& ql, const Q4<R>& qr,
+                                              const Phys<R>& ph) {
+  const R half = (R) 0.5;
+  const R one = (R) 1;
+  R rl = fmax(ql.r, ph.smallr);
+  R ul = ql.u;
+  R pl = fmax(ql.p, rl * ph.smallp);
+  R rr = fmax(qr.r, ph.smallr);
+  R ur = qr.u;
+  R pr = fmax(qr.p, rr * ph.smallp);
+  R cl = ph.gamma * pl * rl;
+  R cr = ph.gamma * pr * rr;
+
+  // First guess and Newton iteration for p* (riemann.c:145-201), in double.
+  const double dpl = pl, dpr = pr, dcl = cl, dcr = cr, dul = ul, dur = ur;
+  double wl0 = sqrt(dcl);
+  double wr0 = sqrt(dcr);
+  double pst = fmax(((wr0 * dpl + wl0 * dpr) + wl0 * wr0 * (dul - dur)) / (wl0 + wr0), 0.0);
+  const double g6 = ph.gamma6_d;
+  for (int it = 0; it < ph.niter; it++) {
+    double wwl = sqrt(dcl * (1.0 + g6 * (pst - dpl) / dpl));
+    double wwr = sqrt(dcr * (1.0 + g6 * (pst - dpr) / dpr));
+    double swwl = wwl * wwl;
+    double qlw = 2.0 * wwl * swwl / (swwl + dcl);
+    double qrw = 2.0 * wwr * (wwr * wwr) / ((wwr * wwr) + dcr);
+    double usl = dul - (pst - dpl) / wwl;
+    double usr = dur + (pst - dpr) / wwr;
+    double t = (qrw * qlw / (qrw + qlw) * (usl - usr));
+    double d = (t > (-pst)) ? t : (-pst);
+    pst += d;
+    double rel = d / (pst + ph.smallpp_d);
+    double arel = (rel > 0) ? rel : -rel;
+    if (!(arel > 1e-6)) break;
+  }
+
+  // Star state and sampling at x/t = 0 (riemann.c:211-276).
+  R ps = (R) pst;
+  R wr = sqrt(cr * (one + (R) g6 * (ps - pr) / pr));
+  R wl = sqrt(cl * (one + (R) g6 * (ps - pl) / pl));
+  R ustar = half * (ul + (pl - ps) / wl + ur - (pr - ps) / wr);
+  bool left = ustar > (R) 0;
+  R sg = left ? (R) 1 : (R) -1;
+  R ro = left ? rl : rr;
+  R uo = left ? ul : ur;
+  R po = left ? pl : pr;
+  R wo = left ? wl : wr;
+  R co = sqrt(fabs(ph.gamma * po / ro));
+  co = fmax(ph.smallc, co);
+  R rstar = ro / (one + ro * (po - ps) / (wo * wo));
+  rstar = fmax(rstar, ph.smallr);
+  R cstar = sqrt(fabs(ph.gamma * ps / rstar));
+  cstar = fmax(ph.smallc, cstar);
+  R spout = co - sg * uo;
+  R spin = cstar - sg * ustar;
+  R ushock = wo / ro - sg * uo;
+  if (ps >= po) {
+    spin = ushock;
+    spout = ushock;
+  }
+  R scr = fmax(spout - spin, ph.smallc + fabs(spout + spin));
+  R frac = (one + (spout + spin) / scr) * half;
+  frac = fmax((R) 0, fmin(one, frac));
+  R gr, gu, gp;
+  if (spout < (R) 0) {
+    gr = ro;
+    gu = uo;
+    gp = po;
+  } else if (spin > (R) 0) {
+    gr = rstar;
+    gu = ustar;
+    gp = ps;
+  } else {
+    gr = frac * rstar + (one - frac) * ro;
+    gu = frac * ustar + (one - frac) * uo;
+    gp = frac * ps + (one - frac) * po;
+  }
+  R gv = left ? ql.v : qr.v;
+
+  // cmpflx.c:67-90
+  Q4<R> f;
+  R m = gr * gu;
+  f.r = m;
+  f.u = m * gu + gp;
+  f.v = m * gv;
+  R ekin = half * gr * (gu * gu + gv * gv);
+  R etot = gp * ph.entho + ekin;
+  f.p = gu * (etot + gp);
+  return f;
+}
+
+// conservar.c:147-197: U_i + (F_{i-1/2} - F_{i+1/2}) * dtdx, accumulated in
+// double (exactly the reference expression when R = double).
+template <typename R>
+__device__ __forceinline__ double update1(R u, R fl, R fr, double dtdx) {
+  return (double) u + ((double) fl - (double) fr) * dtdx;
+}
+
+}  // namespace hydro

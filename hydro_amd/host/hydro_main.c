@@ -1,0 +1,151 @@
+/*
+ * hydro -- the HydroC99 driver over libhydro_amd.
+ *
+ *   hydro -i deck.nml [-v] [-dtlog FILE] [-dump FILE] [-native [-chunk K]]
+ *
+ * Default mode calls the reference call surface step by step exactly as
+ * HydroC99_2DMpi/Src/main.c:244-364 does (dt on even steps, halved at step 0,
+ * hydro_godunov(1) on even steps and (2) on odd ones, t += dt, VTK output on
+ * noutput / dtoutput, the "--> step=" line with MC/s, the average MC/s over
+ * steps > 5).  -native runs the same loop on the GPU (hydro_amd_grid_run) in
+ * chunks of K steps, with one host round trip per chunk.
+ *
+ * -dtlog writes "step t dt" per step with %.17g; -dump writes the final state
+ * as an FP64 binary dump (hydro_amd_dump).
+ */
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "../../include/hydro_amd.h"
+
+static double now(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+int main(int argc, char **argv) {
+  hydroparam_t H;
+  hydrovar_t Hv;
+  hydrowork_t Hw_godunov, Hw_deltat;
+  hydrovarwork_t Hvw_godunov, Hvw_deltat;
+  const char *dtlog_path = NULL, *dump_path = NULL;
+  int native = 0, chunk = 100;
+  char **pass = (char **) calloc((size_t) argc + 1, sizeof(char *));
+  int npass = 0;
+  pass[npass++] = argv[0];
+  for (int n = 1; n < argc; n++) {
+    if (strcmp(argv[n], "-dtlog") == 0 && n + 1 < argc) dtlog_path = argv[++n];
+    else if (strcmp(argv[n], "-dump") == 0 && n + 1 < argc) dump_path = argv[++n];
+    else if (strcmp(argv[n], "-native") == 0) native = 1;
+    else if (strcmp(argv[n], "-chunk") == 0 && n + 1 < argc) chunk = atoi(argv[++n]);
+    else pass[npass++] = argv[n];
+  }
+  if (chunk < 1) chunk = 1;
+  memset(functim, 0, sizeof(functim));
+  process_args(npass, pass, &H);
+  free(pass);
+  hydro_init(&H, &Hv);
+  fprintf(stdout, "Hydro starts in double precision.\n");
+  fprintf(stdout, "Hydro: libhydro_amd on gfx950 (%s loop)\n", native ? "device" : "host");
+
+  FILE *dtlog = dtlog_path ? fopen(dtlog_path, "w") : NULL;
+  int nvtk = 0, time_output = 0;
+  double next_output_time = 0;
+  if (H.dtoutput > 0) {
+    time_output = 1;
+    next_output_time += H.dtoutput;
+  }
+  if (H.dtoutput > 0 || H.noutput > 0) vtkfile(++nvtk, H, &Hv);
+  fprintf(stdout, "Hydro starts main loop.\n");
+  allocate_work_space(H.nxyt, H, &Hw_godunov, &Hvw_godunov);
+  compute_deltat_init_mem(H, &Hw_deltat, &Hvw_deltat);
+
+  double dt = 0, mcsavg = 0, mcsmin = FLT_MAX, mcsmax = 0, mcssig = 0;
+  long nmcs = 0;
+  const double t_start = now();
+  hydro_amd_grid *g = native ? hydro_amd_surface_grid(&Hv) : NULL;
+  while ((H.t < H.tend) && (H.nstep < H.nstepmax)) {
+    double t0 = now();
+    int steps = 1;
+    double dts[1] = {0};
+    double *chunk_dt = NULL;
+    if (!native) {
+      if ((H.nstep % 2) == 0) {
+        dt = 0;
+        compute_deltat(&dt, H, &Hw_deltat, &Hv, &Hvw_deltat);
+        if (H.nstep == 0) {
+          dt = dt / 2.0;
+          fprintf(stdout, "Hydro computes initial deltat: %le\n", dt);
+        }
+      }
+      hydro_godunov((H.nstep % 2) == 0 ? 1 : 2, dt, H, &Hv, &Hw_godunov, &Hvw_godunov);
+      dts[0] = dt;
+      chunk_dt = dts;
+    } else {
+      int want = chunk;
+      if (H.noutput > 0 && !time_output) want = H.noutput - (H.nstep % H.noutput);
+      if (time_output) want = 1;  /* output times are only known step by step */
+      if (want > H.nstepmax - H.nstep) want = H.nstepmax - H.nstep;
+      chunk_dt = (double *) malloc(sizeof(double) * (size_t) want);
+      hydro_amd_grid_set_limits(g, H.tend, H.nstepmax);
+      steps = hydro_amd_grid_run(g, want, chunk_dt);
+      if (steps < 0) {
+        fprintf(stderr, "hydro: %s\n", hydro_amd_last_error());
+        return 1;
+      }
+      if (steps == 0) {
+        free(chunk_dt);
+        break;
+      }
+    }
+    double t1 = now();
+    double mcs = (double) H.globnx * H.globny * steps / (t1 - t0) / 1e6;
+    for (int s = 0; s < steps; s++) {
+      dt = chunk_dt[s];
+      H.nstep++;
+      H.t += dt;
+      if (dtlog) fprintf(dtlog, "%d %.17g %.17g\n", H.nstep, H.t, dt);
+      char out[64] = "";
+      if (time_output == 0 && H.noutput > 0) {
+        if ((H.nstep % H.noutput) == 0) {
+          vtkfile(++nvtk, H, &Hv);
+          snprintf(out, sizeof(out), " [%04d]", nvtk);
+        }
+      } else if (time_output == 1 && H.t >= next_output_time) {
+        vtkfile(++nvtk, H, &Hv);
+        next_output_time += H.dtoutput;
+        snprintf(out, sizeof(out), " [%04d]", nvtk);
+      }
+      fprintf(stdout, "--> step=%4d, %12.5e, %10.5e %.3lf MC/s%s\n", H.nstep, H.t, dt, mcs, out);
+      if (H.nstep > 5) {
+        mcsavg += mcs;
+        nmcs++;
+        if (mcsmin > mcs) mcsmin = mcs;
+        if (mcsmax < mcs) mcsmax = mcs;
+        mcssig += mcs * mcs;
+      }
+    }
+    if (native) free(chunk_dt);
+    fflush(stdout);
+  }
+  const double elaps = now() - t_start;
+  if (dtlog) fclose(dtlog);
+  if (dump_path && hydro_amd_dump(dump_path, H, &Hv) != 0)
+    fprintf(stderr, "hydro: cannot write %s\n", dump_path);
+  deallocate_work_space(H.nxyt, H, &Hw_godunov, &Hvw_godunov);
+  compute_deltat_clean_mem(H, &Hw_deltat, &Hvw_deltat);
+  hydro_finish(H, &Hv);
+  fprintf(stdout, "Hydro ends in %.3lfs.\n", elaps);
+  if (nmcs > 0) {
+    mcsavg /= nmcs;
+    mcssig = sqrt(fabs(mcssig / nmcs - mcsavg * mcsavg));
+    fprintf(stdout, "Average MC/s: %lf min %lf, max %lf sig %lf\n", mcsavg, mcsmin, mcsmax,
+            mcssig);
+  }
+  return 0;
+}

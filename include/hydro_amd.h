@@ -1,0 +1,106 @@
+/*
+ * hydro_amd.h -- handle-based C ABI of libhydro_amd beside the reference call
+ * surface (hydro_surface.h).
+ *
+ * The reference has no API beyond its call surface; these entry points exist
+ * for callers the surface cannot serve: several grids per process, a run of
+ * many steps queued on the GPU without a host round trip per step, batched
+ * grids advanced per launch, mixed precision and the serving loop.  Each one
+ * states which reference function it generalises.
+ *
+ * Conventions: plain pointers and sizes only; int return 0 = success, < 0 =
+ * error with a message from hydro_amd_last_error(); state arrays on the host
+ * are the reference's [4][ny+4][nx+4] FP64 layout (slot 0 rho, 1 rho*u,
+ * 2 rho*v, 3 E, x fastest, two ghost layers).  Calls on one handle must not
+ * overlap; different handles may be used from different threads.
+ */
+#ifndef HYDRO_AMD_H
+#define HYDRO_AMD_H
+
+#include "hydro_surface.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HYDRO_AMD_FP64 0   /* FP64 storage and arithmetic (bitwise the reference's) */
+#define HYDRO_AMD_MIXED 1  /* FP32 storage / slopes / traces / fluxes; FP64 Newton,
+                              update accumulation and CFL reduction */
+
+/* Run parameters: the physics/scheme fields of hydroparam_t
+ * (HydroC99_2DMpi/Src/parametres.h:55-95) plus precision. */
+typedef struct {
+  int nx, ny;
+  double dx, gamma, courant_factor, smallc, smallr, slope_type;
+  int niter_riemann, iorder, scheme;
+  int boundary_left, boundary_right, boundary_down, boundary_up;
+  double tend;
+  int nstepmax;
+  int precision;
+} hydro_amd_params;
+
+typedef struct hydro_amd_grid hydro_amd_grid;
+
+const char *hydro_amd_last_error(void);
+/* Defaults of HydroC99_2DMpi/Src/parametres.c:64-109 with precision FP64. */
+void hydro_amd_params_default(hydro_amd_params *p);
+void hydro_amd_params_from_surface(const hydroparam_t *H, hydro_amd_params *p);
+
+/* hydro_init + allocate_work_space + compute_deltat_init_mem for one grid
+ * (HydroC99_2DMpi/Src/hydro_funcs.c:51-76, :165-293; compute_deltat.c:147):
+ * GPU state, ping-pong buffer, CFL slots and run control.  State starts 0. */
+hydro_amd_grid *hydro_amd_grid_create(const hydro_amd_params *p);
+void hydro_amd_grid_destroy(hydro_amd_grid *g);
+/* Initial conditions of hydro_funcs.c:79-146 (testcase 0, 1, 2) built on the
+ * GPU; init_blast puts energy e at interior cell (bx, by) over the same
+ * background (rho 1, u = v = 0, E = 1e-5).  Both reset t, nstep and dt. */
+int hydro_amd_grid_init(hydro_amd_grid *g, int testcase);
+int hydro_amd_grid_init_blast(hydro_amd_grid *g, int bx, int by, double e);
+/* Whole-array copies in the reference layout (FP32 storage converts). */
+int hydro_amd_grid_upload(hydro_amd_grid *g, const double *uold);
+int hydro_amd_grid_download(hydro_amd_grid *g, double *uold);
+/* hydro_godunov (hydro_godunov.c:63), make_boundary (make_boundary.c:146)
+ * and compute_deltat (compute_deltat.c:166) on one grid. */
+int hydro_amd_grid_godunov(hydro_amd_grid *g, int idim, double dt);
+int hydro_amd_grid_make_boundary(hydro_amd_grid *g, int idim);
+int hydro_amd_grid_compute_deltat(hydro_amd_grid *g, double *dt);
+/* The time loop of HydroC99_2DMpi/Src/main.c:244-298 run on the GPU: up to
+ * max_steps steps while t < tend and nstep < nstepmax, dt recomputed on even
+ * steps and halved at step 0, sweep order alternating.  Steps are queued as a
+ * replayed HIP graph; dt_log (host, may be NULL, max_steps entries) receives
+ * the dt of each step taken.  Returns the number of steps taken (>= 0). */
+int hydro_amd_grid_run(hydro_amd_grid *g, int max_steps, double *dt_log);
+/* Enqueue the same loop without waiting (steps fixed at enqueue time); pair
+ * with hydro_amd_grid_sync. */
+int hydro_amd_grid_run_async(hydro_amd_grid *g, int max_steps);
+int hydro_amd_grid_sync(hydro_amd_grid *g);
+int hydro_amd_grid_state(hydro_amd_grid *g, double *t, int *nstep, double *dt);
+int hydro_amd_grid_set_state(hydro_amd_grid *g, double t, int nstep, double dt);
+int hydro_amd_grid_set_limits(hydro_amd_grid *g, double tend, int nstepmax);
+/* Sweep launch shape: rows per y-sweep thread and waves per y workgroup. */
+int hydro_amd_grid_set_tune(hydro_amd_grid *g, int yseg, int ywaves);
+/* Kernel timing with HIP events recorded on the grid's own stream around
+ * every x-sweep, y-sweep, ghost-fill and CFL launch (graph replay is off
+ * while enabled).  set_profile resets the sums; profile returns summed
+ * milliseconds and launch counts in the order x sweep, y sweep, boundary,
+ * CFL (arrays of 4). */
+int hydro_amd_grid_set_profile(hydro_amd_grid *g, int on);
+int hydro_amd_grid_profile(hydro_amd_grid *g, double *ms, long long *n);
+/* Device bytes held by the grid. */
+long long hydro_amd_grid_bytes(const hydro_amd_grid *g);
+
+/* Host mirror of the reference surface (hydro_surface.h): refresh Hv->uold
+ * from the GPU, or push host edits of Hv->uold to the GPU. */
+int hydro_amd_sync_host(hydrovar_t *Hv);
+int hydro_amd_sync_device(hydrovar_t *Hv);
+/* The grid behind a surface uold (NULL if Hv was not made by hydro_init). */
+hydro_amd_grid *hydro_amd_surface_grid(hydrovar_t *Hv);
+/* FP64 binary dump of the interior (the VTK dump of vtkfile.c is Float32):
+ * "HYDROF64", int32 nx, ny, nvar, nstep, double t, double 0, then
+ * double[nvar][ny][nx].  Refreshes the host mirror first. */
+int hydro_amd_dump(const char *path, const hydroparam_t H, hydrovar_t *Hv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
