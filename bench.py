@@ -36,13 +36,15 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 BYTES_PER_CELL_SWEEP = {0: 64, 1: 32}  # FP64 / FP32 storage: 4 vars read + 4 written
 
 
-def gpu_bench(nx, ny, steps, warmup, precision):
+def gpu_bench(nx, ny, steps, warmup, precision, autotune=True):
     import hydro_amd
     p = hydro_amd.Params(nx=nx, ny=ny, dx=1.0 / nx, courant_factor=0.8, niter_riemann=10,
                          iorder=2, slope_type=1.0, tend=1e30, nstepmax=2**31 - 1,
                          precision=precision)
     g = hydro_amd.Grid(p)
     g.init(0)
+    # launch shapes picked on this grid before timing (the state is untouched)
+    tune = g.autotune(reps=2) if autotune else None
     if warmup:
         g.run(warmup, log=False)
     g.set_profile(True)
@@ -63,6 +65,7 @@ def gpu_bench(nx, ny, steps, warmup, precision):
     avg_s = ms / n / 1e3
     achieved = per_launch_bytes / avg_s / 1e9
     return {
+        "tune": tune,
         "value": cells * steps / elapsed,
         "ms_per_step": elapsed / steps * 1e3,
         "elapsed_s": elapsed,
@@ -131,6 +134,7 @@ def main():
     ap.add_argument("--ny", type=int, default=0)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=8)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -139,7 +143,7 @@ def main():
         sys.exit(2)
     ny = a.ny or a.nx
     prec = 0 if a.precision == "fp64" else 1
-    r = gpu_bench(a.nx, ny, a.steps, a.warmup, prec)
+    r = gpu_bench(a.nx, ny, a.steps, a.warmup, prec, autotune=not a.no_autotune)
     line = {
         "metric": METRIC,
         "value": r["value"],
@@ -157,7 +161,7 @@ def main():
                                f"{'FP64' if prec == 0 else 'mixed FP32/FP64'}, "
                                "CFL 0.8, reflective walls",
                    "nx": a.nx, "ny": ny, "testcase": 0, "steps_timed": a.steps,
-                   "first_timed_step": a.warmup},
+                   "first_timed_step": a.warmup, "launch_shape": r["tune"]},
         "roofline": r["roofline"],
         "kernels": r["kernels"],
     }

@@ -107,8 +107,9 @@ def lib() -> C.CDLL:
                                                  C.POINTER(C.c_double)]),
                        ("hydro_amd_grid_set_state", [G, C.c_double, C.c_int, C.c_double]),
                        ("hydro_amd_grid_set_limits", [G, C.c_double, C.c_int]),
-                       ("hydro_amd_grid_set_tune", [G, C.c_int, C.c_int]),
+                       ("hydro_amd_grid_set_tune", [G, C.c_int, C.c_int, C.c_int, C.c_int]),
                        ("hydro_amd_grid_set_profile", [G, C.c_int]),
+                       ("hydro_amd_grid_autotune", [G, C.c_int, C.c_void_p]),
                        ("hydro_amd_grid_profile", [G, _dp, C.POINTER(C.c_longlong)])]:
         fn = getattr(L, name)
         fn.argtypes = args
@@ -210,8 +211,14 @@ class Grid:
     def set_limits(self, tend: float, nstepmax: int) -> None:
         _check(lib().hydro_amd_grid_set_limits(self._h, tend, nstepmax))
 
-    def set_tune(self, yseg: int, ywaves: int) -> None:
-        _check(lib().hydro_amd_grid_set_tune(self._h, yseg, ywaves))
+    def set_tune(self, yseg: int = 64, ywaves: int = 4, xwaves: int = 4, xiter: int = 8) -> None:
+        _check(lib().hydro_amd_grid_set_tune(self._h, yseg, ywaves, xwaves, xiter))
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Pick the fastest sweep launch shapes for this grid (state unchanged)."""
+        best = (C.c_int * 4)()
+        _check(lib().hydro_amd_grid_autotune(self._h, reps, C.cast(best, C.c_void_p)))
+        return dict(zip(("yseg", "ywaves", "xwaves", "xiter"), list(best)))
 
     def set_profile(self, on: bool) -> None:
         _check(lib().hydro_amd_grid_set_profile(self._h, int(on)))

@@ -66,7 +66,7 @@ struct hydro_amd_grid {
   long state_ver = 1;
   long cour_ver = 0;     // state version whose CFL maxima the slots hold (0: none)
   bool cour_dirty = false;
-  SweepTune tune = {64, 4};
+  SweepTune tune = {64, 4, 4, 8};
   // host mirror of the run control, valid between calls
   double t = 0, dt = 0;
   int nstep = 0;
@@ -631,12 +631,116 @@ extern "C" int hydro_amd_grid_profile(hydro_amd_grid* g, double* ms, long long* 
   return 0;
 }
 
-extern "C" int hydro_amd_grid_set_tune(hydro_amd_grid* g, int yseg, int ywaves) {
+// ---- autotuner ------------------------------------------------------------------------
+
+// Times one sweep launch shape: `reps` launches reading buffer 0 and writing
+// buffer 1's interior (buffer 0, the state, is untouched; buffer 1 is scratch
+// between steps), HIP events on the grid's stream.  Returns ms per launch.
+static double time_sweep(hydro_amd_grid* g, int idim, const SweepTune& t, int reps) {
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess) return 1e30;
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void) hipEventDestroy(a);
+    return 1e30;
+  }
+  auto launch = [&] {
+    if (g->prec == HYDRO_AMD_FP64)
+      launch_sweep<double>(idim, g->geom, g->ph, g->vd[0], g->vd[1], 1e-12, nullptr, nullptr, t,
+                           g->st);
+    else
+      launch_sweep<float>(idim, g->geom, g->ph, g->vf[0], g->vf[1], 1e-12, nullptr, nullptr, t,
+                          g->st);
+  };
+  launch();  // warm
+  (void) hipEventRecord(a, g->st);
+  for (int r = 0; r < reps; r++) launch();
+  (void) hipEventRecord(b, g->st);
+  float ms = 0;
+  if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess)
+    ms = 1e30f;
+  (void) hipEventDestroy(a);
+  (void) hipEventDestroy(b);
+  return ms / reps;
+}
+
+namespace {
+struct TuneKey {
+  int nx, ny, prec;
+  bool operator<(const TuneKey& o) const {
+    return nx != o.nx ? nx < o.nx : ny != o.ny ? ny < o.ny : prec < o.prec;
+  }
+};
+std::map<TuneKey, SweepTune> g_tuned;
+std::mutex g_tuned_mu;
+}  // namespace
+
+extern "C" int hydro_amd_grid_autotune(hydro_amd_grid* g, int reps, int* best) {
+  if (!g) return fail("null grid");
+  if (reps < 1) reps = 1;
+  const TuneKey key{g->p.nx, g->p.ny, g->prec};
+  {
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end() && reps > 0) {
+      g->tune = it->second;
+      drop_graph(g);
+      if (best) {
+        best[0] = g->tune.yseg; best[1] = g->tune.ywaves;
+        best[2] = g->tune.xwaves; best[3] = g->tune.xiter;
+      }
+      return 0;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(g->st));
+  SweepTune cur = g->tune, bt = g->tune;
+  // y sweep: rows per lane (the slab height each column march covers) x
+  // waves per workgroup
+  double by = 1e30;
+  for (int yseg : {16, 32, 64, 128, 256, 512}) {
+    if (yseg > 4 * g->p.ny && yseg != 16) continue;
+    for (int yw : {1, 2, 4}) {
+      SweepTune t = cur;
+      t.yseg = yseg;
+      t.ywaves = yw;
+      const double ms = time_sweep(g, 2, t, reps);
+      if (ms < by) { by = ms; bt.yseg = yseg; bt.ywaves = yw; }
+    }
+  }
+  // x sweep: waves per workgroup x segments walked per wave
+  double bx = 1e30;
+  for (int xw : {1, 2, 4}) {
+    for (int xi : {1, 2, 4, 8, 16, 32}) {
+      SweepTune t = cur;
+      t.xwaves = xw;
+      t.xiter = xi;
+      const double ms = time_sweep(g, 1, t, reps);
+      if (ms < bx) { bx = ms; bt.xwaves = xw; bt.xiter = xi; }
+    }
+  }
+  HIPCHK(hipGetLastError());
+  g->tune = bt;
+  drop_graph(g);
+  {
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    g_tuned[key] = bt;
+  }
+  if (best) {
+    best[0] = bt.yseg; best[1] = bt.ywaves; best[2] = bt.xwaves; best[3] = bt.xiter;
+  }
+  return 0;
+}
+
+extern "C" int hydro_amd_grid_set_tune(hydro_amd_grid* g, int yseg, int ywaves, int xwaves,
+                                       int xiter) {
   if (!g) return fail("null grid");
   if (yseg < 1 || yseg > 1 << 20) return fail("yseg out of range");
   if (ywaves < 1 || ywaves > 4) return fail("ywaves must be 1..4");
+  if (xwaves < 1 || xwaves > 4) return fail("xwaves must be 1..4");
+  if (xiter < 1 || xiter > 4096) return fail("xiter out of range");
   g->tune.yseg = yseg;
   g->tune.ywaves = ywaves;
+  g->tune.xwaves = xwaves;
+  g->tune.xiter = xiter;
   drop_graph(g);
   return 0;
 }

@@ -98,7 +98,7 @@ template <typename S, bool ST1, bool COUR>
 __global__ __launch_bounds__(256) void k_sweep_x(GridGeom g, Phys<S> ph, double dx,
                                                  double dt_val, const RunCtl* ctl,
                                                  GridView<S> in, GridView<S> out,
-                                                 uint64_t* cour, int nseg) {
+                                                 uint64_t* cour, int nseg, int xiter) {
   double dt = dt_val;
   if (ctl) {
     if (!ctl->active) return;  // grid-uniform
@@ -107,50 +107,53 @@ __global__ __launch_bounds__(256) void k_sweep_x(GridGeom g, Phys<S> ph, double 
   const double dtdx = dt / dx;  // hydro_godunov.c:101
   const TraceK<S> tk = make_tracek<S>((S) dtdx, ph.scheme);
   const int lane = threadIdx.x & 63;
-  const long task = (long) blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const long jrow = task / nseg;
-  const int seg = (int) (task - jrow * nseg);
-  const bool rowok = jrow < g.ny;
-  const long j = rowok ? jrow + 2 : 2;
-  const int i = kXSeg * seg + lane;  // padded column of this lane
-  const int ic = i < g.nx + 4 ? i : g.nx + 3;
-  const long o = j * g.pitch + ic;
+  const int wpb = blockDim.x >> 6;
+  const long ntask = (long) nseg * g.ny;
+  double cx = 0.0, cy = 0.0;
+  // A workgroup walks wpb * xiter consecutive row segments, wpb at a time.
+  for (int it = 0; it < xiter; it++) {
+    const long task = ((long) blockIdx.x * xiter + it) * wpb + (threadIdx.x >> 6);
+    const bool ok = task < ntask;
+    const long jrow = ok ? task / nseg : 0;
+    const int seg = ok ? (int) (task - jrow * nseg) : 0;
+    const long j = jrow + 2;
+    const int i = kXSeg * seg + lane;  // padded column of this lane
+    const int ic = i < g.nx + 4 ? i : g.nx + 3;
+    const long o = j * g.pitch + ic;
 
-  const S U0 = in.p[0][o], U1 = in.p[1][o], U2 = in.p[2][o], U3 = in.p[3][o];
-  const Prim<S> pc = prim_cell<S>(U0, U1, U2, U3, ph);
-  Q4<S> dq = {(S) 0, (S) 0, (S) 0, (S) 0};
-  if (ph.iorder != 1) {
-    const Q4<S> qm = from_left4(pc.q), qp = from_right4(pc.q);
-    dq = slope4<S, ST1>(qm, pc.q, qp, ph.slope_type);
-  }
-  Q4<S> qxm, qxp;
-  trace_cell<S>(pc.q, dq, pc.c, tk, qxm, qxp);
-  const Q4<S> qr = from_right4(qxp);
-  // flux through the face between this lane's cell and the next one
-  const Q4<S> fr = riemann_flux<S>(qxm, qr, ph);
-  const Q4<S> fl = from_left4(fr);
-
-  const bool wr = rowok && lane >= 2 && lane <= kXSeg + 1 && i <= g.nx + 1;
-  const double n0 = update1<S>(U0, fl.r, fr.r, dtdx);
-  const double n1 = update1<S>(U1, fl.u, fr.u, dtdx);
-  const double n2 = update1<S>(U2, fl.v, fr.v, dtdx);
-  const double n3 = update1<S>(U3, fl.p, fr.p, dtdx);
-  if (wr) {
-    out.p[0][o] = (S) n0;
-    out.p[1][o] = (S) n1;
-    out.p[2][o] = (S) n2;
-    out.p[3][o] = (S) n3;
-  }
-  if (COUR) {
-    double cx = 0.0, cy = 0.0;
-    if (wr) {
-      S ax, ay;
-      courant_cell<S>((S) n0, (S) n1, (S) n2, (S) n3, ph, ax, ay);
-      cx = ax;
-      cy = ay;
+    const S U0 = in.p[0][o], U1 = in.p[1][o], U2 = in.p[2][o], U3 = in.p[3][o];
+    const Prim<S> pc = prim_cell<S>(U0, U1, U2, U3, ph);
+    Q4<S> dq = {(S) 0, (S) 0, (S) 0, (S) 0};
+    if (ph.iorder != 1) {
+      const Q4<S> qm = from_left4(pc.q), qp = from_right4(pc.q);
+      dq = slope4<S, ST1>(qm, pc.q, qp, ph.slope_type);
     }
-    commit_courant(cx, cy, cour);
+    Q4<S> qxm, qxp;
+    trace_cell<S>(pc.q, dq, pc.c, tk, qxm, qxp);
+    const Q4<S> qr = from_right4(qxp);
+    // flux through the face between this lane's cell and the next one
+    const Q4<S> fr = riemann_flux<S>(qxm, qr, ph);
+    const Q4<S> fl = from_left4(fr);
+
+    const bool wr = ok && lane >= 2 && lane <= kXSeg + 1 && i <= g.nx + 1;
+    const double n0 = update1<S>(U0, fl.r, fr.r, dtdx);
+    const double n1 = update1<S>(U1, fl.u, fr.u, dtdx);
+    const double n2 = update1<S>(U2, fl.v, fr.v, dtdx);
+    const double n3 = update1<S>(U3, fl.p, fr.p, dtdx);
+    if (wr) {
+      out.p[0][o] = (S) n0;
+      out.p[1][o] = (S) n1;
+      out.p[2][o] = (S) n2;
+      out.p[3][o] = (S) n3;
+      if (COUR) {
+        S ax, ay;
+        courant_cell<S>((S) n0, (S) n1, (S) n2, (S) n3, ph, ax, ay);
+        cx = maxm(cx, (double) ax);
+        cy = maxm(cy, (double) ay);
+      }
+    }
   }
+  if (COUR) commit_courant(cx, cy, cour);
 }
 
 // ---- y sweep ---------------------------------------------------------------
@@ -431,10 +434,14 @@ void launch_sweep(int idim, const GridGeom& g, const PhysArgs& pa, GridView<S> i
   if (idim == 1) {
     const int nseg = (int) cdiv(g.nx, kXSeg);
     const long waves = (long) nseg * g.ny;
-    const dim3 grid(cdiv(waves, 4)), blk(256);
+    const int xw = tune.xwaves > 0 ? tune.xwaves : 4;
+    int xiter = tune.xiter > 0 ? tune.xiter : 8;
+    // keep at least ~4 workgroups per CU before walking several segments
+    while (xiter > 1 && cdiv(waves, (long) xw * xiter) < 1024) xiter >>= 1;
+    const dim3 grid(cdiv(waves, (long) xw * xiter)), blk(64 * xw);
 #define HX(A, B) \
   hipLaunchKernelGGL((k_sweep_x<S, A, B>), grid, blk, 0, st, g, ph, pa.dx, dt_val, ctl, in, out, \
-                     cour, nseg)
+                     cour, nseg, xiter)
     if (st1) { if (c) HX(true, true); else HX(true, false); }
     else { if (c) HX(false, true); else HX(false, false); }
 #undef HX

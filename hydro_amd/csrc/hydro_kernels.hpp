@@ -45,6 +45,8 @@ constexpr int kXSeg = 60;
 struct SweepTune {
   int yseg;     // rows per y-sweep thread (column march length)
   int ywaves;   // waves per y-sweep workgroup (1..4)
+  int xwaves;   // waves per x-sweep workgroup (1..4)
+  int xiter;    // row segments each x-sweep wave walks
 };
 
 // ---- launchers (hydro_kernels.hip) -----------------------------------------

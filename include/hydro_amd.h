@@ -77,8 +77,16 @@ int hydro_amd_grid_sync(hydro_amd_grid *g);
 int hydro_amd_grid_state(hydro_amd_grid *g, double *t, int *nstep, double *dt);
 int hydro_amd_grid_set_state(hydro_amd_grid *g, double t, int nstep, double dt);
 int hydro_amd_grid_set_limits(hydro_amd_grid *g, double tend, int nstepmax);
-/* Sweep launch shape: rows per y-sweep thread and waves per y workgroup. */
-int hydro_amd_grid_set_tune(hydro_amd_grid *g, int yseg, int ywaves);
+/* Sweep launch shape (results do not depend on it): rows each y-sweep lane
+ * marches, waves per y-sweep workgroup (1..4), waves per x-sweep workgroup
+ * (1..4) and row segments each x-sweep wave walks. */
+int hydro_amd_grid_set_tune(hydro_amd_grid *g, int yseg, int ywaves, int xwaves, int xiter);
+/* Autotuner: times the candidate launch shapes of both sweeps on this grid
+ * (reps launches each, HIP events; the sweeps read the state and write only
+ * the scratch buffer, so the state is unchanged), keeps the fastest and
+ * caches it per (nx, ny, precision) for the process.  best[4] (may be NULL)
+ * receives yseg, ywaves, xwaves, xiter. */
+int hydro_amd_grid_autotune(hydro_amd_grid *g, int reps, int *best);
 /* Kernel timing with HIP events recorded on the grid's own stream around
  * every x-sweep, y-sweep, ghost-fill and CFL launch (graph replay is off
  * while enabled).  set_profile resets the sums; profile returns summed

@@ -173,20 +173,39 @@ def test_surface_calls_on_uploaded_state(port):
         assert g.compute_deltat() == port.compute_deltat(po, u)
 
 
-@pytest.mark.parametrize("yseg,ywaves", [(1, 1), (7, 2), (64, 4), (1000, 3)])
-def test_tiling_independence(port, yseg, ywaves):
-    """Results do not depend on the y-sweep segment length or workgroup shape."""
+@pytest.mark.parametrize("yseg,ywaves,xwaves,xiter", [(1, 1, 1, 1), (7, 2, 3, 5), (64, 4, 4, 8),
+                                                     (1000, 3, 2, 64)])
+def test_tiling_independence(port, yseg, ywaves, xwaves, xiter):
+    """Results do not depend on the sweep launch shapes."""
     p = hydro_amd.Params(**{**C0, "nx": 130, "ny": 150, "dx": 1 / 130.0})
     po = to_oracle(p, testcase=0)
     ur = port.init(po)
     rdt, _ = port.run(po, ur, 10)
     with hydro_amd.Grid(p) as g:
-        g.set_tune(yseg, ywaves)
+        g.set_tune(yseg, ywaves, xwaves, xiter)
         g.init(0)
         dts = g.run(10)
         u = g.download()
     assert np.array_equal(dts, rdt)
-    assert_same(u, ur, f"yseg={yseg} ywaves={ywaves}")
+    assert_same(u, ur, f"tune {yseg} {ywaves} {xwaves} {xiter}")
+
+
+def test_autotune_keeps_state_and_results(port):
+    """The autotuner only writes scratch: the state and the run are unchanged."""
+    p = hydro_amd.Params(**{**C0, "nx": 300, "ny": 200, "dx": 1 / 300.0})
+    po = to_oracle(p, testcase=1)
+    ur = port.init(po)
+    rdt, _ = port.run(po, ur, 8)
+    with hydro_amd.Grid(p) as g:
+        g.init(1)
+        before = g.download()
+        best = g.autotune(reps=1)
+        assert set(best) == {"yseg", "ywaves", "xwaves", "xiter"}
+        assert np.array_equal(g.download(), before)
+        dts = g.run(8)
+        u = g.download()
+    assert np.array_equal(dts, rdt)
+    assert_same(u, ur, f"after autotune {best}")
 
 
 def test_mid_size_against_oracle(port):
