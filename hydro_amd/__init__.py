@@ -74,6 +74,40 @@ class Params:
         return (4, self.ny + 4, self.nx + 4)
 
 
+class HydroParam(C.Structure):
+    """hydroparam_t of include/hydro_surface.h (HydroC99_2DMpi/Src/parametres.h:55-95)."""
+    _fields_ = [("prt", C.c_int), ("t", C.c_double), ("tend", C.c_double), ("nstep", C.c_int),
+                ("nstepmax", C.c_int), ("noutput", C.c_int), ("dtoutput", C.c_double),
+                ("imin", C.c_int), ("imax", C.c_int), ("jmin", C.c_int), ("jmax", C.c_int),
+                ("nx", C.c_int), ("ny", C.c_int), ("nxt", C.c_int), ("nyt", C.c_int),
+                ("nxyt", C.c_int), ("nxystep", C.c_int), ("nproc", C.c_int), ("mype", C.c_int),
+                ("globnx", C.c_int), ("globny", C.c_int), ("box", C.c_int * 8),
+                ("nvar", C.c_int), ("dx", C.c_double), ("gamma", C.c_double),
+                ("courant_factor", C.c_double), ("smallc", C.c_double), ("smallr", C.c_double),
+                ("niter_riemann", C.c_int), ("iorder", C.c_int), ("slope_type", C.c_double),
+                ("scheme", C.c_int), ("boundary_right", C.c_int), ("boundary_left", C.c_int),
+                ("boundary_down", C.c_int), ("boundary_up", C.c_int), ("testCase", C.c_int)]
+
+
+class HydroVar(C.Structure):
+    _fields_ = [("uold", C.POINTER(C.c_double))]
+
+
+def process_args(args) -> HydroParam:
+    """The library's process_args (a C99 host function; no GPU needed)."""
+    argv = (C.c_char_p * (len(args) + 1))(*([b"hydro"] + [a.encode() for a in args]))
+    H = HydroParam()
+    lib().process_args(len(args) + 1, argv, C.byref(H))
+    return H
+
+
+def vtkfile(step: int, H: HydroParam, uold: np.ndarray) -> None:
+    """The library's vtkfile on a host array (writes ./Dep/..., ./Hydro.pvd)."""
+    u = np.ascontiguousarray(uold, dtype=np.float64)
+    hv = HydroVar(u.ctypes.data_as(C.POINTER(C.c_double)))
+    lib().vtkfile(step, H, C.byref(hv))
+
+
 _LIB = None
 
 
@@ -116,6 +150,10 @@ def lib() -> C.CDLL:
         fn.restype = C.c_int
     L.hydro_amd_grid_bytes.argtypes = [G]
     L.hydro_amd_grid_bytes.restype = C.c_longlong
+    L.process_args.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(HydroParam)]
+    L.process_args.restype = None
+    L.vtkfile.argtypes = [C.c_int, HydroParam, C.POINTER(HydroVar)]
+    L.vtkfile.restype = None
     _LIB = L
     return L
 

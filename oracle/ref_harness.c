@@ -127,3 +127,22 @@ void ref_riemann_line(int nface, int nl, double smallr, double smallc, double ga
   free(w.goon);
   free(blk);
 }
+
+/* The reference's process_args alone (no allocation): reals as ref_get_reals,
+ * ints: nx, ny, nxystep, nstepmax, testcase, niter_riemann, iorder, scheme,
+ * boundary left, right, down, up, noutput, nvar. */
+void ref_parse(const char *nml, double *reals, int *ints) {
+  hydroparam_t P;
+  char *argv[3];
+  argv[0] = "hydro";
+  argv[1] = "-i";
+  argv[2] = (char *) nml;
+  process_args(3, argv, &P);
+  reals[0] = P.t; reals[1] = P.tend; reals[2] = P.dx; reals[3] = P.gamma;
+  reals[4] = P.courant_factor; reals[5] = P.smallc; reals[6] = P.smallr; reals[7] = P.slope_type;
+  reals[8] = P.dtoutput;
+  ints[0] = P.nx; ints[1] = P.ny; ints[2] = P.nxystep; ints[3] = P.nstepmax; ints[4] = P.testCase;
+  ints[5] = P.niter_riemann; ints[6] = P.iorder; ints[7] = P.scheme; ints[8] = P.boundary_left;
+  ints[9] = P.boundary_right; ints[10] = P.boundary_down; ints[11] = P.boundary_up;
+  ints[12] = P.noutput; ints[13] = P.nvar;
+}
