@@ -150,6 +150,31 @@ def lib() -> C.CDLL:
         fn.restype = C.c_int
     L.hydro_amd_grid_bytes.argtypes = [G]
     L.hydro_amd_grid_bytes.restype = C.c_longlong
+    B = C.c_void_p
+    L.hydro_amd_batch_create.argtypes = [C.c_int, C.c_longlong]
+    L.hydro_amd_batch_create.restype = B
+    L.hydro_amd_batch_destroy.argtypes = [B]
+    L.hydro_amd_batch_destroy.restype = None
+    for name, args in [("hydro_amd_batch_add", [B, P]),
+                       ("hydro_amd_batch_remove", [B, C.c_int]),
+                       ("hydro_amd_batch_init", [B, C.c_int, C.c_int]),
+                       ("hydro_amd_batch_init_blast", [B, C.c_int, C.c_int, C.c_int, C.c_double]),
+                       ("hydro_amd_batch_upload", [B, C.c_int, _dp]),
+                       ("hydro_amd_batch_download", [B, C.c_int, _dp]),
+                       ("hydro_amd_batch_run", [B, C.c_int]),
+                       ("hydro_amd_batch_run_async", [B, C.c_int]),
+                       ("hydro_amd_batch_sync", [B]),
+                       ("hydro_amd_batch_query", [B, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                  C.c_void_p, C.c_void_p, C.c_int]),
+                       ("hydro_amd_batch_set_tune", [B, C.c_int, C.c_int, C.c_int]),
+                       ("hydro_amd_batch_autotune", [B, C.c_int, C.c_void_p])]:
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    for name in ("hydro_amd_batch_bytes_needed", "hydro_amd_batch_bytes_used"):
+        getattr(L, name).restype = C.c_longlong
+    L.hydro_amd_batch_bytes_needed.argtypes = [B, C.c_int, C.c_int]
+    L.hydro_amd_batch_bytes_used.argtypes = [B]
     L.process_args.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(HydroParam)]
     L.process_args.restype = None
     L.vtkfile.argtypes = [C.c_int, HydroParam, C.POINTER(HydroVar)]
@@ -272,3 +297,103 @@ class Grid:
     @property
     def nbytes(self) -> int:
         return int(lib().hydro_amd_grid_bytes(self._h))
+
+
+class Batch:
+    """Many independent grids advanced per launch (hydro_amd_batch_*)."""
+
+    def __init__(self, precision: int = FP64, budget_bytes: int = 0):
+        self.precision = precision
+        self._h = lib().hydro_amd_batch_create(precision, int(budget_bytes))
+        if not self._h:
+            raise HydroError(last_error())
+        self.params = {}
+
+    def close(self) -> None:
+        if self._h:
+            lib().hydro_amd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, p: Params) -> int:
+        """Admit an instance; raises HydroError (e.g. over budget)."""
+        i = lib().hydro_amd_batch_add(self._h, C.byref(p.c()))
+        if i < 0:
+            raise HydroError(last_error())
+        self.params[i] = p
+        return i
+
+    def try_add(self, p: Params):
+        """Admit an instance, or return None when it does not fit the budget."""
+        i = lib().hydro_amd_batch_add(self._h, C.byref(p.c()))
+        if i < 0:
+            return None
+        self.params[i] = p
+        return i
+
+    def remove(self, i: int) -> None:
+        _check(lib().hydro_amd_batch_remove(self._h, i))
+        self.params.pop(i, None)
+
+    def bytes_needed(self, nx: int, ny: int) -> int:
+        return int(lib().hydro_amd_batch_bytes_needed(self._h, nx, ny))
+
+    @property
+    def bytes_used(self) -> int:
+        return int(lib().hydro_amd_batch_bytes_used(self._h))
+
+    def init(self, i: int, testcase: int) -> None:
+        _check(lib().hydro_amd_batch_init(self._h, i, testcase))
+
+    def init_blast(self, i: int, bx: int, by: int, e: float) -> None:
+        _check(lib().hydro_amd_batch_init_blast(self._h, i, bx, by, e))
+
+    def upload(self, i: int, u: np.ndarray) -> None:
+        _check(lib().hydro_amd_batch_upload(self._h, i, np.ascontiguousarray(u, np.float64)))
+
+    def download(self, i: int) -> np.ndarray:
+        u = np.empty(self.params[i].shape())
+        _check(lib().hydro_amd_batch_download(self._h, i, u))
+        return u
+
+    def run(self, steps: int) -> None:
+        _check(lib().hydro_amd_batch_run(self._h, steps))
+
+    def run_async(self, steps: int) -> None:
+        _check(lib().hydro_amd_batch_run_async(self._h, steps))
+
+    def sync(self) -> None:
+        _check(lib().hydro_amd_batch_sync(self._h))
+
+    def query(self):
+        """{id: (t, nstep, dt, more_steps)} for every live instance."""
+        cap = max(len(self.params), 1)
+        ids = np.zeros(cap, np.int32)
+        t = np.zeros(cap)
+        n = np.zeros(cap, np.int32)
+        dt = np.zeros(cap)
+        act = np.zeros(cap, np.int32)
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        k = _check(lib().hydro_amd_batch_query(self._h, ptr(ids), ptr(t), ptr(n), ptr(dt),
+                                               ptr(act), cap))
+        return {int(ids[j]): (float(t[j]), int(n[j]), float(dt[j]), bool(act[j]))
+                for j in range(min(k, cap))}
+
+    def set_tune(self, waves: int, xiter: int, yseg: int) -> None:
+        _check(lib().hydro_amd_batch_set_tune(self._h, waves, xiter, yseg))
+
+    def autotune(self, reps: int = 2) -> dict:
+        best = (C.c_int * 3)()
+        _check(lib().hydro_amd_batch_autotune(self._h, reps, C.cast(best, C.c_void_p)))
+        return dict(zip(("waves", "xiter", "yseg"), list(best)))

@@ -1,0 +1,235 @@
+// Batched kernels: many independent grids (instances) of uneven sizes advanced
+// per launch.  The arithmetic per cell is the single-grid code
+// (hydro_sweeps.hpp), so every instance's result is bit-identical to running
+// it alone.
+//
+// A step is two phases.  In each phase every instance does one sweep, x or y
+// depending on its own step parity (even steps x then y, odd steps y then x,
+// HydroC99_2DMpi/Src/main.c:285-291), so a phase's task table mixes x tasks
+// of some instances with y tasks of others.  Workgroups are persistent: the
+// grid is sized to the GPU, and each workgroup walks the task table with a
+// stride.  A task is one workgroup's worth of one instance:
+//   x task: wpb * xiter consecutive 60-cell row segments (each wave walks xiter)
+//   y task: 64 columns x (wpb * yseg) rows (each wave marches yseg rows)
+// so the CFL reduction of a task never mixes instances.  Run control (t, dt,
+// nstep, active) and CFL slots are per instance, on the device.
+#include "hydro_batch.hpp"
+#include "hydro_sweeps.hpp"
+
+namespace hydro {
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_batch_sweep(const InstDev<S>* inst, const BTask* tasks,
+                                                     int ntask, int phase, int xiter, int yseg) {
+  const int wpb = blockDim.x >> 6;
+  const int wave = threadIdx.x >> 6;
+  for (int t = blockIdx.x; t < ntask; t += gridDim.x) {
+    const BTask tk = tasks[t];
+    const InstDev<S>& d = inst[tk.inst];
+    if (!d.ctl->active) continue;  // workgroup-uniform
+    const double dtdx = d.ctl->dt / d.dx;
+    const TraceK<S> tr = make_tracek<S>((S) dtdx, d.ph.scheme);
+    const GridView<S> in = d.buf[phase], out = d.buf[1 - phase];
+    // the second sweep of an odd step is an x sweep; it produces the CFL
+    // maxima the next (even) step's dt needs
+    const bool cour = phase == 1 && tk.kind == 1;
+    const bool st1 = d.ph.slope_type == (S) 1;
+    double cx = 0.0, cy = 0.0;
+    if (tk.kind == 1) {
+      const long ntot = (long) d.nseg * d.g.ny;
+      for (int it = 0; it < xiter; it++) {
+        const long s = (long) tk.a + (long) it * wpb + wave;
+        const bool ok = s < ntot && s < (long) tk.a + (long) wpb * xiter;
+        const long jrow = ok ? s / d.nseg : 0;
+        const int seg = ok ? (int) (s - jrow * d.nseg) : 0;
+        if (st1) {
+          if (cour) x_segment<S, true, true>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
+          else x_segment<S, true, false>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
+        } else {
+          if (cour) x_segment<S, false, true>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
+          else x_segment<S, false, false>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
+        }
+      }
+    } else {
+      const int col = tk.a * 64 + (threadIdx.x & 63);
+      const bool colok = col < d.g.nx;
+      const long i = 2 + (colok ? col : d.g.nx - 1);
+      const int j0 = 2 + (tk.b * wpb + wave) * yseg;
+      if (j0 < d.g.ny + 2) {  // wave-uniform
+        const int j1 = min(j0 + yseg, d.g.ny + 2);
+        if (st1) y_march<S, true, false>(d.g, d.ph, tr, dtdx, in, out, colok, i, j0, j1, cx, cy);
+        else y_march<S, false, false>(d.g, d.ph, tr, dtdx, in, out, colok, i, j0, j1, cx, cy);
+      }
+    }
+    if (cour) commit_courant(cx, cy, d.cour, kInstSlots, t % kInstSlots);
+  }
+}
+
+// Ghost fill of one instance per task (kind = direction), from buffer `phase`
+// into both buffers.
+template <typename S>
+__global__ __launch_bounds__(256) void k_batch_boundary(const InstDev<S>* inst,
+                                                        const BTask* tasks, int ntask,
+                                                        int phase) {
+  for (int t = blockIdx.x; t < ntask; t += gridDim.x) {
+    const BTask tk = tasks[t];
+    const InstDev<S>& d = inst[tk.inst];
+    if (!d.ctl->active) continue;
+    const int idim = tk.kind;
+    const int nl = (idim == 1) ? d.g.ny : d.g.nx;
+    const GridView<S> src = d.buf[phase], dst = d.buf[1 - phase];
+    for (int k = threadIdx.x; k < 4 * nl; k += blockDim.x) {
+      const int line = k % nl, v = k / nl;
+      boundary_line<S>(idim, d.g, d.bl, d.br, d.bd, d.bu, src.p[v], dst.p[v], v, line);
+    }
+  }
+}
+
+// CFL maxima of buffer 0 for the listed instances (admission / step 0).
+template <typename S>
+__global__ __launch_bounds__(256) void k_batch_courant(const InstDev<S>* inst, const int* ids,
+                                                       int nids) {
+  for (int t = blockIdx.x; t < nids * 64; t += gridDim.x) {
+    const InstDev<S>& d = inst[ids[t / 64]];
+    const int part = t % 64;
+    const long n = (long) d.g.nx * d.g.ny;
+    double cx = 0.0, cy = 0.0;
+    for (long c = (long) part * blockDim.x + threadIdx.x; c < n; c += 64L * blockDim.x) {
+      const long jj = c / d.g.nx, ii = c - jj * d.g.nx;
+      const long o = (jj + 2) * d.g.pitch + ii + 2;
+      S ax, ay;
+      courant_cell<S>(d.buf[0].p[0][o], d.buf[0].p[1][o], d.buf[0].p[2][o], d.buf[0].p[3][o],
+                      d.ph, ax, ay);
+      cx = maxm(cx, (double) ax);
+      cy = maxm(cy, (double) ay);
+    }
+    commit_courant(cx, cy, d.cour, kInstSlots, part % kInstSlots);
+  }
+}
+
+// main.c:244-260 for every listed instance: one wave per instance.
+template <typename S>
+__global__ __launch_bounds__(64) void k_batch_ctl_begin(const InstDev<S>* inst, const int* ids,
+                                                        int nids) {
+  for (int k = blockIdx.x; k < nids; k += gridDim.x) {
+    const InstDev<S>& d = inst[ids[k]];
+    RunCtl* c = d.ctl;
+    const double t = c->t, tend = c->tend;
+    const int nstep = c->nstep, nstepmax = c->nstepmax;
+    const bool act = (t < tend) && (nstep < nstepmax);
+    if (act && (nstep % 2) == 0) {
+      double dt = wave_reduce_dt(d.cour, kInstSlots, d.cf, d.dx, d.smallc);
+      if (nstep == 0) dt = dt / 2.0;
+      if (threadIdx.x == 0) c->dt = dt;
+    }
+    if (threadIdx.x == 0) c->active = act ? 1 : 0;
+  }
+}
+
+// main.c:297-298
+template <typename S>
+__global__ void k_batch_ctl_end(const InstDev<S>* inst, const int* ids, int nids) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nids) return;
+  RunCtl* c = inst[ids[k]].ctl;
+  if (!c->active) return;
+  c->nstep += 1;
+  c->t += c->dt;
+}
+
+__global__ void k_batch_set_ctl(RunCtl* c, double t, double dt, double tend, int nstep,
+                                int nstepmax, int active) {
+  if (threadIdx.x != 0) return;
+  c->t = t;
+  c->dt = dt;
+  c->tend = tend;
+  c->nstep = nstep;
+  c->nstepmax = nstepmax;
+  c->active = active;
+  c->log = nullptr;
+  c->log_cap = 0;
+  c->log_base = 0;
+}
+
+__global__ void k_fill_u64(uint64_t* p, long n) {
+  for (long k = (long) blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (long) gridDim.x * blockDim.x)
+    p[k] = 0;
+}
+
+static inline unsigned cdivu(long a, long b) { return (unsigned) ((a + b - 1) / b); }
+
+void launch_batch_set_ctl(RunCtl* c, double t, double dt, double tend, int nstep, int nstepmax,
+                          int active, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_set_ctl, dim3(1), dim3(64), 0, st, c, t, dt, tend, nstep, nstepmax,
+                     active);
+}
+
+void launch_zero_u64(uint64_t* p, long n, hipStream_t st) {
+  if (n <= 0) return;
+  long nb = (n + 255) / 256;
+  if (nb > 1024) nb = 1024;
+  hipLaunchKernelGGL(k_fill_u64, dim3((unsigned) nb), dim3(256), 0, st, p, n);
+}
+
+template <typename S>
+int batch_sweep_occupancy(int waves) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_sweep<S>, 64 * waves, 0) !=
+      hipSuccess)
+    return 2;
+  return nb > 0 ? nb : 1;
+}
+template int batch_sweep_occupancy<double>(int);
+template int batch_sweep_occupancy<float>(int);
+
+template <typename S>
+void launch_batch_sweep(const InstDev<S>* inst, const BTask* tasks, int ntask, int phase,
+                        const BatchTune& tn, int nblocks, hipStream_t st) {
+  if (ntask <= 0) return;
+  const int nb = ntask < nblocks ? ntask : nblocks;
+  hipLaunchKernelGGL(k_batch_sweep<S>, dim3(nb), dim3(64 * tn.waves), 0, st, inst, tasks, ntask,
+                     phase, tn.xiter, tn.yseg);
+}
+
+template <typename S>
+void launch_batch_boundary(const InstDev<S>* inst, const BTask* tasks, int ntask, int phase,
+                           int nblocks, hipStream_t st) {
+  if (ntask <= 0) return;
+  const int nb = ntask < nblocks ? ntask : nblocks;
+  hipLaunchKernelGGL(k_batch_boundary<S>, dim3(nb), dim3(256), 0, st, inst, tasks, ntask, phase);
+}
+
+template <typename S>
+void launch_batch_courant(const InstDev<S>* inst, const int* ids, int nids, hipStream_t st) {
+  if (nids <= 0) return;
+  long nb = 64L * nids;
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(k_batch_courant<S>, dim3((unsigned) nb), dim3(256), 0, st, inst, ids, nids);
+}
+
+template <typename S>
+void launch_batch_ctl(const InstDev<S>* inst, const int* ids, int nids, bool begin,
+                      hipStream_t st) {
+  if (nids <= 0) return;
+  if (begin) {
+    const int nb = nids < 1024 ? nids : 1024;
+    hipLaunchKernelGGL(k_batch_ctl_begin<S>, dim3(nb), dim3(64), 0, st, inst, ids, nids);
+  } else {
+    hipLaunchKernelGGL(k_batch_ctl_end<S>, dim3(cdivu(nids, 256)), dim3(256), 0, st, inst, ids,
+                       nids);
+  }
+}
+
+#define INST(S)                                                                                  \
+  template void launch_batch_sweep<S>(const InstDev<S>*, const BTask*, int, int,                \
+                                      const BatchTune&, int, hipStream_t);                        \
+  template void launch_batch_boundary<S>(const InstDev<S>*, const BTask*, int, int, int,        \
+                                         hipStream_t);                                           \
+  template void launch_batch_courant<S>(const InstDev<S>*, const int*, int, hipStream_t);       \
+  template void launch_batch_ctl<S>(const InstDev<S>*, const int*, int, bool, hipStream_t);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace hydro

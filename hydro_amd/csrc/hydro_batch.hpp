@@ -1,0 +1,54 @@
+// Internal interface of the batched kernels (hydro_batch.hip).
+#pragma once
+#include "hydro_kernels.hpp"
+#include "hydro_numerics.hpp"
+
+namespace hydro {
+
+constexpr int kInstSlots = 64;  // CFL max slots per instance
+
+// One instance as the batched kernels see it (device array, rebuilt by the
+// host whenever the set of instances changes).
+template <typename S>
+struct InstDev {
+  GridGeom g;
+  Phys<S> ph;
+  double dx, cf, smallc;  // dt = cf*dx / max(cx, cy, smallc)
+  GridView<S> buf[2];     // state in buf[0] between steps
+  RunCtl* ctl;
+  uint64_t* cour;         // 2 * kInstSlots
+  int bl, br, bd, bu;     // boundary types
+  int nseg;               // 60-cell segments per row
+};
+
+// One workgroup's work: kind 1 = x (a = first segment), 2 = y (a = 64-column
+// block, b = row-segment group); for ghost fills kind is the direction.
+struct BTask {
+  int inst, kind, a, b;
+};
+
+struct BatchTune {
+  int waves;  // waves per workgroup (1..4)
+  int xiter;  // segments each wave walks in an x task
+  int yseg;   // rows each wave marches in a y task
+};
+
+template <typename S>
+void launch_batch_sweep(const InstDev<S>* inst, const BTask* tasks, int ntask, int phase,
+                        const BatchTune& tn, int nblocks, hipStream_t st);
+template <typename S>
+void launch_batch_boundary(const InstDev<S>* inst, const BTask* tasks, int ntask, int phase,
+                           int nblocks, hipStream_t st);
+template <typename S>
+void launch_batch_courant(const InstDev<S>* inst, const int* ids, int nids, hipStream_t st);
+template <typename S>
+void launch_batch_ctl(const InstDev<S>* inst, const int* ids, int nids, bool begin,
+                      hipStream_t st);
+
+void launch_batch_set_ctl(RunCtl* c, double t, double dt, double tend, int nstep, int nstepmax,
+                          int active, hipStream_t st);
+void launch_zero_u64(uint64_t* p, long n, hipStream_t st);
+template <typename S>
+int batch_sweep_occupancy(int waves);  // resident workgroups per CU
+
+}  // namespace hydro

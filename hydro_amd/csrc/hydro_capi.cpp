@@ -40,6 +40,17 @@ static int fail(const char* fmt, ...) {
 
 extern "C" const char* hydro_amd_last_error(void) { return g_err.c_str(); }
 
+// For the other translation units of the library (hydro_batch.cpp, ...).
+int hydro_amd_set_error(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return -1;
+}
+
 // ---- grid ------------------------------------------------------------------------
 
 // Row pitch is a multiple of 32 elements and rows start 30 elements into the

@@ -1,0 +1,308 @@
+// Task-level device code of the fused sweeps, shared by the single-grid
+// kernels (hydro_kernels.hip) and the batched kernels (hydro_batch.hip), so
+// both run exactly the same arithmetic.
+//
+// * x sweep: one 64-lane wave owns 64 consecutive cells of a row and updates
+//   the middle 60 (kXSeg).  Each lane turns its cell into primitives;
+//   neighbours come from DPP wave shifts (v_mov_b32_dpp wave_shl/wave_shr), so
+//   slope, trace, Riemann solve, flux and update all stay in registers.
+// * y sweep: one lane owns one column and marches down a segment of rows with
+//   a three-cell register window (primitives, one carried trace state, one
+//   carried flux).  A wave covers 64 adjacent columns, so every load/store is
+//   one contiguous row piece.
+// * Both read one state buffer and write the other (ping-pong).
+// * CFL maxima of the written state can be folded into the epilogue: reduced
+//   wave -> workgroup -> one 64-bit atomic max per workgroup into one of a
+//   set of slots; the bit pattern of a non-negative double orders like its
+//   value, so the integer max is the exact floating max.
+#pragma once
+#include "hydro_kernels.hpp"
+#include "hydro_numerics.hpp"
+
+namespace hydro {
+
+// ---- lane exchange -----------------------------------------------------------
+
+// lane l receives the value of lane l-1 (DPP wave_shr:1); lane 0 keeps its own.
+__device__ __forceinline__ double from_left(double x) {
+  int lo = __double2loint(x), hi = __double2hiint(x);
+  lo = __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false);
+  hi = __builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+// lane l receives the value of lane l+1 (DPP wave_shl:1); lane 63 keeps its own.
+__device__ __forceinline__ double from_right(double x) {
+  int lo = __double2loint(x), hi = __double2hiint(x);
+  lo = __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false);
+  hi = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float from_left(float x) {
+  int b = __float_as_int(x);
+  b = __builtin_amdgcn_update_dpp(b, b, 0x138, 0xf, 0xf, false);
+  return __int_as_float(b);
+}
+__device__ __forceinline__ float from_right(float x) {
+  int b = __float_as_int(x);
+  b = __builtin_amdgcn_update_dpp(b, b, 0x130, 0xf, 0xf, false);
+  return __int_as_float(b);
+}
+template <typename R>
+__device__ __forceinline__ Q4<R> from_left4(const Q4<R>& a) {
+  return Q4<R>{from_left(a.r), from_left(a.u), from_left(a.v), from_left(a.p)};
+}
+template <typename R>
+__device__ __forceinline__ Q4<R> from_right4(const Q4<R>& a) {
+  return Q4<R>{from_right(a.r), from_right(a.u), from_right(a.v), from_right(a.p)};
+}
+
+// ---- CFL maximum -------------------------------------------------------------
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = maxm(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Workgroup max of (cx, cy), then one atomic max each into slot `slot` of
+// cour[0..nslot) / cour[nslot..2 nslot).  Every thread of the workgroup must
+// call it (it has barriers); it can be called again by the same workgroup.
+__device__ __forceinline__ void commit_courant(double cx, double cy, uint64_t* cour, int nslot,
+                                               int slot) {
+  __shared__ double sx[16], sy[16];
+  cx = wave_max(cx);
+  cy = wave_max(cy);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sx[w] = cx;
+    sy[w] = cy;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    double mx = sx[0], my = sy[0];
+    for (int k = 1; k < nw; k++) {
+      mx = maxm(mx, sx[k]);
+      my = maxm(my, sy[k]);
+    }
+    atomicMax((unsigned long long*) &cour[slot], (unsigned long long) __double_as_longlong(mx));
+    atomicMax((unsigned long long*) &cour[nslot + slot],
+              (unsigned long long) __double_as_longlong(my));
+  }
+  __syncthreads();
+}
+
+// Max over nslot slot pairs (clearing them), then compute_deltat.c:197:
+// dt = cfl*dx / MAX(cx, MAX(cy, smallc)).  Called by one whole wave; every
+// lane returns the same dt.
+__device__ __forceinline__ double wave_reduce_dt(uint64_t* cour, int nslot, double cf, double dx,
+                                                 double smallc) {
+  double cx = 0.0, cy = 0.0;
+  const int lane = threadIdx.x & 63;
+  for (int s = lane; s < nslot; s += 64) {
+    cx = maxm(cx, __longlong_as_double((long long) cour[s]));
+    cy = maxm(cy, __longlong_as_double((long long) cour[nslot + s]));
+    cour[s] = 0;
+    cour[nslot + s] = 0;
+  }
+  cx = wave_max(cx);
+  cy = wave_max(cy);
+  return cf * dx / maxm(cx, maxm(cy, smallc));
+}
+
+// ---- x sweep: one 60-cell segment of one row per wave ----------------------
+
+template <typename S, bool ST1, bool COUR>
+__device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
+                                          const TraceK<S>& tk, double dtdx,
+                                          const GridView<S>& in, const GridView<S>& out,
+                                          bool ok, long jrow, int seg, double& cx, double& cy) {
+  const int lane = threadIdx.x & 63;
+  const long j = jrow + 2;
+  const int i = kXSeg * seg + lane;  // padded column of this lane
+  const int ic = i < g.nx + 4 ? i : g.nx + 3;
+  const long o = j * g.pitch + ic;
+
+  const S U0 = in.p[0][o], U1 = in.p[1][o], U2 = in.p[2][o], U3 = in.p[3][o];
+  const Prim<S> pc = prim_cell<S>(U0, U1, U2, U3, ph);
+  Q4<S> dq = {(S) 0, (S) 0, (S) 0, (S) 0};
+  if (ph.iorder != 1) {
+    const Q4<S> qm = from_left4(pc.q), qp = from_right4(pc.q);
+    dq = slope4<S, ST1>(qm, pc.q, qp, ph.slope_type);
+  }
+  Q4<S> qxm, qxp;
+  trace_cell<S>(pc.q, dq, pc.c, tk, qxm, qxp);
+  const Q4<S> qr = from_right4(qxp);
+  // flux through the face between this lane's cell and the next one
+  const Q4<S> fr = riemann_flux<S>(qxm, qr, ph);
+  const Q4<S> fl = from_left4(fr);
+
+  const bool wr = ok && lane >= 2 && lane <= kXSeg + 1 && i <= g.nx + 1;
+  const double n0 = update1<S>(U0, fl.r, fr.r, dtdx);
+  const double n1 = update1<S>(U1, fl.u, fr.u, dtdx);
+  const double n2 = update1<S>(U2, fl.v, fr.v, dtdx);
+  const double n3 = update1<S>(U3, fl.p, fr.p, dtdx);
+  if (wr) {
+    out.p[0][o] = (S) n0;
+    out.p[1][o] = (S) n1;
+    out.p[2][o] = (S) n2;
+    out.p[3][o] = (S) n3;
+    if (COUR) {
+      S ax, ay;
+      courant_cell<S>((S) n0, (S) n1, (S) n2, (S) n3, ph, ax, ay);
+      cx = maxm(cx, (double) ax);
+      cy = maxm(cy, (double) ay);
+    }
+  }
+}
+
+// ---- y sweep: one column, rows [j0, j1) ----------------------------------------
+
+template <typename S>
+struct Cell {
+  S u0, u1, u2, u3;  // conserved, sweep slot order
+};
+
+template <typename S>
+__device__ __forceinline__ Cell<S> load_y(const GridView<S>& in, long o) {
+  // gather with the IU/IV swap of conservar.c:106-109
+  return Cell<S>{in.p[0][o], in.p[2][o], in.p[1][o], in.p[3][o]};
+}
+
+template <typename S, bool ST1, bool COUR>
+__device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
+                                        const TraceK<S>& tk, double dtdx,
+                                        const GridView<S>& in, const GridView<S>& out,
+                                        bool colok, long i, int j0, int j1, double& cx,
+                                        double& cy) {
+  const long P = g.pitch;
+  const S st = ph.slope_type;
+  const bool o1 = ph.iorder == 1;
+
+  // prologue: cells j0-2 .. j0+1
+  Cell<S> ua = load_y(in, (long) (j0 - 2) * P + i);
+  Cell<S> ub = load_y(in, (long) (j0 - 1) * P + i);
+  Cell<S> uc = load_y(in, (long) j0 * P + i);
+  Cell<S> ud = load_y(in, (long) (j0 + 1) * P + i);
+  Prim<S> pa = prim_cell<S>(ua.u0, ua.u1, ua.u2, ua.u3, ph);
+  Prim<S> pb = prim_cell<S>(ub.u0, ub.u1, ub.u2, ub.u3, ph);
+  Prim<S> pcc = prim_cell<S>(uc.u0, uc.u1, uc.u2, uc.u3, ph);
+  Prim<S> pd = prim_cell<S>(ud.u0, ud.u1, ud.u2, ud.u3, ph);
+  const Q4<S> z = {(S) 0, (S) 0, (S) 0, (S) 0};
+  Q4<S> qxm_b, qxp_b, qxm_c, qxp_c;
+  trace_cell<S>(pb.q, o1 ? z : slope4<S, ST1>(pa.q, pb.q, pcc.q, st), pb.c, tk, qxm_b, qxp_b);
+  trace_cell<S>(pcc.q, o1 ? z : slope4<S, ST1>(pb.q, pcc.q, pd.q, st), pcc.c, tk, qxm_c, qxp_c);
+  Q4<S> fprev = riemann_flux<S>(qxm_b, qxp_c, ph);  // face (j0-1 | j0)
+
+  // window: cell C = row j-2 (being updated), D = row j-1; carried: qxm of C,
+  // flux through the face below C
+  Q4<S> qC = pcc.q, qD = pd.q;
+  S cD = pd.c;
+  Q4<S> qxmC = qxm_c;
+  Cell<S> uC = uc, uD = ud;
+  for (int j = j0 + 2; j <= j1 + 1; j++) {
+    const long oj = (long) j * P + i;
+    const Cell<S> uE = load_y(in, oj);
+    const Prim<S> pE = prim_cell<S>(uE.u0, uE.u1, uE.u2, uE.u3, ph);
+    Q4<S> qxmD, qxpD;
+    trace_cell<S>(qD, o1 ? z : slope4<S, ST1>(qC, qD, pE.q, st), cD, tk, qxmD, qxpD);
+    const Q4<S> fcur = riemann_flux<S>(qxmC, qxpD, ph);  // face (j-2 | j-1)
+    const double n0 = update1<S>(uC.u0, fprev.r, fcur.r, dtdx);
+    const double n1 = update1<S>(uC.u1, fprev.u, fcur.u, dtdx);
+    const double n2 = update1<S>(uC.u2, fprev.v, fcur.v, dtdx);
+    const double n3 = update1<S>(uC.u3, fprev.p, fcur.p, dtdx);
+    const long oc = (long) (j - 2) * P + i;
+    if (colok) {
+      // scatter back with the swap undone (conservar.c:177-180)
+      out.p[0][oc] = (S) n0;
+      out.p[2][oc] = (S) n1;
+      out.p[1][oc] = (S) n2;
+      out.p[3][oc] = (S) n3;
+      if (COUR) {
+        S ax, ay;
+        courant_cell<S>((S) n0, (S) n2, (S) n1, (S) n3, ph, ax, ay);
+        cx = maxm(cx, (double) ax);
+        cy = maxm(cy, (double) ay);
+      }
+    }
+    fprev = fcur;
+    qxmC = qxmD;
+    qC = qD;
+    qD = pE.q;
+    cD = pE.c;
+    uC = uD;
+    uD = uE;
+  }
+}
+
+// ---- ghost cells -------------------------------------------------------------
+
+// make_boundary.c:237-289 (x) and 381-433 (y) for one interior line k and one
+// variable v: the four ghost cells in the reference's order (low side 0, 1,
+// then high side n+2, n+3), so sources that are themselves ghosts (grids
+// narrower than 3 cells, the absorbing high-x source n+2) read exactly what
+// the reference reads.  Types: 1 reflect (normal momentum negated), 2 absorb,
+// any other positive value periodic, <= 0 untouched.  Values go to src and,
+// when dst is non-null, to dst (the other ping-pong buffer).
+template <typename S>
+__device__ __forceinline__ void boundary_line(int idim, const GridGeom& g, int bl, int br,
+                                              int bd, int bu, S* sp, S* dp, int v, int k) {
+  const int nn = (idim == 1) ? g.nx : g.ny;
+  const int normal = (idim == 1) ? 1 : 2;
+  const long line = k + 2;
+  for (int gp = 0; gp < 4; gp++) {
+    const int lo = gp < 2;
+    const int typ = (idim == 1) ? (lo ? bl : br) : (lo ? bd : bu);
+    if (typ <= 0) continue;
+    const int gidx = lo ? gp : nn + gp;  // 0, 1, nn+2, nn+3
+    int sidx;
+    S sign = (S) 1;
+    if (typ == 1) {
+      sidx = lo ? 3 - gidx : 2 * nn + 3 - gidx;
+      if (v == normal) sign = (S) -1;
+    } else if (typ == 2) {
+      // reference source cells: left 2, right nn+2, down 4, up ny+1
+      if (idim == 1) sidx = lo ? 2 : nn + 2;
+      else sidx = lo ? 4 : nn + 1;
+    } else {
+      sidx = lo ? nn + gidx : gidx - nn;
+    }
+    long od, os;
+    if (idim == 1) {
+      od = line * g.pitch + gidx;
+      os = line * g.pitch + sidx;
+    } else {
+      od = (long) gidx * g.pitch + line;
+      os = (long) sidx * g.pitch + line;
+    }
+    const S val = sp[os] * sign;
+    sp[od] = val;
+    if (dp) dp[od] = val;
+  }
+}
+
+// ---- initial state ---------------------------------------------------------------
+
+// hydro_funcs.c:79-128: interior rho=1, u=v=0, E=1e-5, ghosts 0, E=eb at padded
+// column bx for rows [by0, by1).  Element t of (nx+4)*(ny+4).
+template <typename S>
+__device__ __forceinline__ void init_cell(const GridGeom& g, GridView<S> a, GridView<S> b,
+                                          long t, int bx, int by0, int by1, S eb) {
+  const long nxt = g.nx + 4, nyt = g.ny + 4;
+  const long j = t / nxt, i = t - j * nxt;
+  const bool in = i >= 2 && i < nxt - 2 && j >= 2 && j < nyt - 2;
+  S v0 = in ? (S) 1 : (S) 0, v3 = in ? (S) 1e-5 : (S) 0;
+  if (i == bx && j >= by0 && j < by1) v3 = eb;
+  const long o = j * g.pitch + i;
+  a.p[0][o] = v0;
+  a.p[1][o] = (S) 0;
+  a.p[2][o] = (S) 0;
+  a.p[3][o] = v3;
+  if (b.p[0]) {
+    b.p[0][o] = v0;
+    b.p[1][o] = (S) 0;
+    b.p[2][o] = (S) 0;
+    b.p[3][o] = v3;
+  }
+}
+
+}  // namespace hydro

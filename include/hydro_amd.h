@@ -97,6 +97,45 @@ int hydro_amd_grid_profile(hydro_amd_grid *g, double *ms, long long *n);
 /* Device bytes held by the grid. */
 long long hydro_amd_grid_bytes(const hydro_amd_grid *g);
 
+/* ---- batches: many independent grids advanced per launch ----------------
+ * Each instance is a separate run of the reference time loop (own size,
+ * gamma, blast, dt, t, nstep, tend/nstepmax), bit-identical to running it
+ * alone with hydro_amd_grid_run.  One persistent workgroup grid per phase
+ * walks a task table mixing the instances' x and y sweeps; two steps are
+ * replayed as a HIP graph.  Instances are admitted against an HBM budget
+ * (state buffers only) with the stream-ordered allocator, so admission and
+ * retirement never synchronise the device.  All instances share one
+ * precision. */
+typedef struct hydro_amd_batch hydro_amd_batch;
+hydro_amd_batch *hydro_amd_batch_create(int precision, long long budget_bytes);
+void hydro_amd_batch_destroy(hydro_amd_batch *b);
+/* Admit an instance (state zero, t = nstep = 0); returns its id >= 0, or -1
+ * (e.g. over budget) with the reason in hydro_amd_last_error(). */
+int hydro_amd_batch_add(hydro_amd_batch *b, const hydro_amd_params *p);
+int hydro_amd_batch_remove(hydro_amd_batch *b, int id);
+long long hydro_amd_batch_bytes_needed(const hydro_amd_batch *b, int nx, int ny);
+long long hydro_amd_batch_bytes_used(const hydro_amd_batch *b);
+/* hydro_amd_grid_init / init_blast / upload / download for one instance. */
+int hydro_amd_batch_init(hydro_amd_batch *b, int id, int testcase);
+int hydro_amd_batch_init_blast(hydro_amd_batch *b, int id, int bx, int by, double e);
+int hydro_amd_batch_upload(hydro_amd_batch *b, int id, const double *uold);
+int hydro_amd_batch_download(hydro_amd_batch *b, int id, double *uold);
+/* Advance every live instance by up to `steps` steps of its own time loop
+ * (instances that reach tend / nstepmax stop; the rest go on). */
+int hydro_amd_batch_run(hydro_amd_batch *b, int steps);
+int hydro_amd_batch_run_async(hydro_amd_batch *b, int steps);
+int hydro_amd_batch_sync(hydro_amd_batch *b);
+/* Waits, then reports the live instances: ids, t, nstep, dt and whether
+ * another step would open (t < tend && nstep < nstepmax).  Fills up to cap
+ * entries; returns the number of live instances. */
+int hydro_amd_batch_query(hydro_amd_batch *b, int *ids, double *t, int *nstep, double *dt,
+                          int *active, int cap);
+/* Task shape (waves per workgroup 1..4, segments per wave in x tasks, rows per
+ * wave in y tasks) and its autotuner (times the shapes on the live set; state
+ * unchanged).  best[3] receives waves, xiter, yseg. */
+int hydro_amd_batch_set_tune(hydro_amd_batch *b, int waves, int xiter, int yseg);
+int hydro_amd_batch_autotune(hydro_amd_batch *b, int reps, int *best);
+
 /* Host mirror of the reference surface (hydro_surface.h): refresh Hv->uold
  * from the GPU, or push host edits of Hv->uold to the GPU. */
 int hydro_amd_sync_host(hydrovar_t *Hv);
