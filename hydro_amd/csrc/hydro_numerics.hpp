@@ -10,8 +10,8 @@
 //
 // Mixed precision (R = float): storage, primitives, slopes, traces, wave
 // speeds and fluxes are float; the Riemann first guess and Newton pressure
-// iteration run in double (riemann_face), the update accumulates in double
-// (update_cell) and the CFL maximum is kept in double by the callers.
+// iteration run in double (riemann_flux), the update accumulates in double
+// (update1) and the CFL maximum and dt are double in the callers.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -64,6 +64,15 @@ struct Prim {
 template <typename R>
 __device__ __forceinline__ R maxm(R x, R y) { return x > y ? x : y; }  // utils.h:47 MAX
 
+// Division and square root.  FP64: the IEEE operations themselves (what the
+// reference computes).  FP32 (mixed precision only): the hardware reciprocal
+// and square root (v_rcp_f32 / v_sqrt_f32, about 1 ulp each) instead of the
+// correctly rounded multi-instruction sequences.
+__device__ __forceinline__ double xdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float xdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double xsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ float xsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+
 // constoprim.c:62-78 + equation_of_state.c:64-81 for one cell.  U is in sweep
 // slot order (slot 1 = normal momentum).
 template <typename R>
@@ -71,13 +80,13 @@ __device__ __forceinline__ Prim<R> prim_cell(R U0, R U1, R U2, R U3, const Phys<
   Prim<R> o;
   const R half = (R) 0.5;
   R r = maxm(U0, ph.smallr);
-  R u = U1 / r;
-  R v = U2 / r;
+  R u = xdiv(U1, r);
+  R v = xdiv(U2, r);
   R eken = half * (u * u + v * v);
-  R e = U3 / r - eken;
+  R e = xdiv(U3, r) - eken;
   R p = ph.gm1 * r * e;
   p = maxm(p, r * ph.smallp);
-  o.c = sqrt(ph.gamma * p / r);
+  o.c = xsqrt(xdiv(ph.gamma * p, r));
   o.q.r = r;
   o.q.u = u;
   o.q.v = v;
@@ -108,7 +117,7 @@ __device__ __forceinline__ R slope1(R qm, R q0, R qp, R st) {
   } else {
     dl = st * (q0 - qm);
     dr = st * (qp - q0);
-    dc = half * (dl + dr) / st;
+    dc = xdiv(half * (dl + dr), st);
   }
   R sg = (dc > (R) 0) ? (R) 1 : (R) -1;
   R keep = ((dl * dr) <= (R) 0) ? (R) 0 : (R) 1;
@@ -141,8 +150,8 @@ __device__ __forceinline__ TraceK<R> make_tracek(R dtdx, int scheme) {
   k.zeror = (R) 0;
   k.project = (R) 0;
   if (scheme == 1) {
-    k.zerol = -(R) 100 / dtdx;
-    k.zeror = (R) 100 / dtdx;
+    k.zerol = xdiv(-(R) 100, dtdx);
+    k.zeror = xdiv((R) 100, dtdx);
     k.project = (R) 1;
   } else if (scheme == 2) {
     k.project = (R) 1;
@@ -158,12 +167,12 @@ __device__ __forceinline__ void trace_cell(const Q4<R>& q, const Q4<R>& dq, R cc
   const R half = (R) 0.5;
   const R one = (R) 1;
   R csq = cc * cc;
-  R rOcc = q.r / cc;
-  R OrOcc = cc / q.r;
-  R dprcc = dq.p / (q.r * cc);
+  R rOcc = xdiv(q.r, cc);
+  R OrOcc = xdiv(cc, q.r);
+  R dprcc = xdiv(dq.p, q.r * cc);
   R am = half * (dprcc - dq.u) * rOcc;
   R ap = half * (dprcc + dq.u) * rOcc;
-  R a0r = dq.r - dq.p / csq;
+  R a0r = dq.r - xdiv(dq.p, csq);
   R a0v = dq.v;
   R upcc = q.u + cc;
   R umcc = q.u - cc;
@@ -238,30 +247,30 @@ __device__ __forceinline__ Q4<R> riemann_flux(const Q4<R>& ql, const Q4<R>& qr,
 
   // Star state and sampling at x/t = 0 (riemann.c:211-276).
   R ps = (R) pst;
-  R wr = sqrt(cr * (one + (R) g6 * (ps - pr) / pr));
-  R wl = sqrt(cl * (one + (R) g6 * (ps - pl) / pl));
-  R ustar = half * (ul + (pl - ps) / wl + ur - (pr - ps) / wr);
+  R wr = xsqrt(cr * (one + xdiv((R) g6 * (ps - pr), pr)));
+  R wl = xsqrt(cl * (one + xdiv((R) g6 * (ps - pl), pl)));
+  R ustar = half * (ul + xdiv(pl - ps, wl) + ur - xdiv(pr - ps, wr));
   bool left = ustar > (R) 0;
   R sg = left ? (R) 1 : (R) -1;
   R ro = left ? rl : rr;
   R uo = left ? ul : ur;
   R po = left ? pl : pr;
   R wo = left ? wl : wr;
-  R co = sqrt(fabs(ph.gamma * po / ro));
+  R co = xsqrt(fabs(xdiv(ph.gamma * po, ro)));
   co = fmax(ph.smallc, co);
-  R rstar = ro / (one + ro * (po - ps) / (wo * wo));
+  R rstar = xdiv(ro, one + xdiv(ro * (po - ps), wo * wo));
   rstar = fmax(rstar, ph.smallr);
-  R cstar = sqrt(fabs(ph.gamma * ps / rstar));
+  R cstar = xsqrt(fabs(xdiv(ph.gamma * ps, rstar)));
   cstar = fmax(ph.smallc, cstar);
   R spout = co - sg * uo;
   R spin = cstar - sg * ustar;
-  R ushock = wo / ro - sg * uo;
+  R ushock = xdiv(wo, ro) - sg * uo;
   if (ps >= po) {
     spin = ushock;
     spout = ushock;
   }
   R scr = fmax(spout - spin, ph.smallc + fabs(spout + spin));
-  R frac = (one + (spout + spin) / scr) * half;
+  R frac = (one + xdiv(spout + spin, scr)) * half;
   frac = fmax((R) 0, fmin(one, frac));
   R gr, gu, gp;
   if (spout < (R) 0) {
