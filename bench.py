@@ -36,13 +36,13 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 BYTES_PER_CELL_SWEEP = {0: 64, 1: 32}  # FP64 / FP32 storage: 4 vars read + 4 written
 
 
-def gpu_bench(nx, ny, steps, warmup, precision, autotune=True):
+def gpu_bench(nx, ny, steps, warmup, precision, autotune=True, testcase=0):
     import hydro_amd
     p = hydro_amd.Params(nx=nx, ny=ny, dx=1.0 / nx, courant_factor=0.8, niter_riemann=10,
                          iorder=2, slope_type=1.0, tend=1e30, nstepmax=2**31 - 1,
                          precision=precision)
     g = hydro_amd.Grid(p)
-    g.init(0)
+    g.init(testcase)
     # launch shapes picked on this grid before timing (the state is untouched)
     tune = g.autotune(reps=2) if autotune else None
     if warmup:
@@ -79,6 +79,107 @@ def gpu_bench(nx, ny, steps, warmup, precision, autotune=True):
     }
 
 
+def batched_bench(n=512, steps=100):
+    """Config 3: n seeded blast instances (sides log-uniform in [64, 1024]),
+    `steps` steps each: the plain per-grid FP64 loop (one Grid per instance,
+    created, run and freed in turn) against one batch (FP64 and mixed)."""
+    import hydro_amd
+    from hydro_amd import workloads as W
+    out = {"instances": n, "steps": steps,
+           "cell_updates": W.cell_updates(W.batch_instances(n, steps=steps))}
+    runs = W.batch_instances(n, steps=steps)
+    t0 = time.perf_counter()
+    for r in runs:
+        with hydro_amd.Grid(r.params) as g:
+            g.init_blast(r.bx, r.by, r.e)
+            g.run(steps, log=False)
+    t_plain = time.perf_counter() - t0
+    out["plain_fp64_per_grid"] = {"seconds": t_plain, "solves_per_s": n / t_plain}
+    for name, prec in (("batched_fp64", hydro_amd.FP64), ("batched_mixed", hydro_amd.MIXED)):
+        runs = W.batch_instances(n, steps=steps, precision=prec)
+        with hydro_amd.Batch(prec) as b:
+            ids = [b.add(r.params) for r in runs]
+            for i, r in zip(ids, runs):
+                b.init_blast(i, r.bx, r.by, r.e)
+            tune = b.autotune(reps=1)
+            b.sync()
+            t0 = time.perf_counter()
+            b.run(steps)
+            t = time.perf_counter() - t0
+            q = b.query()
+            assert all(v[1] == steps for v in q.values())
+        out[name] = {"seconds": t, "solves_per_s": n / t, "cell_updates_per_s":
+                     out["cell_updates"] / t, "tune": tune,
+                     "speedup_vs_plain_fp64": t_plain / t}
+    return out
+
+
+def serving_bench(n=32, rate=20.0, chunk=20):
+    """Config 4: n runs (sides log-uniform in [128, 4096], t_end 0.1 or 500
+    steps) arriving as a Poisson process; mixed precision served by the
+    continuous-batching loop against the plain per-grid FP64 loop (FIFO, one
+    run at a time) on the same requests.  Latency = finished - arrived."""
+    import numpy as np
+    import hydro_amd
+    from hydro_amd import workloads as W
+    runs = W.serving_requests(n, rate=rate, precision=hydro_amd.MIXED)
+    srv = hydro_amd.Server(precision=hydro_amd.MIXED, budget_bytes=200 * 10**9,
+                           chunk_steps=chunk, keep_results=True)
+    t_start = hydro_amd.now()
+    ids = []
+    k = 0
+    while k < n or srv.pump() > 0:
+        now = hydro_amd.now() - t_start
+        while k < n and runs[k].arrival <= now:
+            ids.append(srv.submit(runs[k].params, runs[k].bx, runs[k].by, runs[k].e))
+            k += 1
+        if k < n and srv.counts()["running"] == 0 and srv.counts()["queued"] == 0:
+            time.sleep(max(0.0, min(0.01, runs[k].arrival - now)))
+    info = [srv.info(i) for i in ids]
+    assert all(x[0] == srv.DONE for x in info), [x[0] for x in info]
+    lat = np.array([x[3] - x[1] for x in info])
+    span = max(x[3] for x in info) - t_start
+    res = {"requests": n, "rate_per_s": rate, "chunk_steps": chunk,
+           "served_mixed": {"solves_per_s": n / span, "seconds": span,
+                            "p50_s": float(np.percentile(lat, 50)),
+                            "p99_s": float(np.percentile(lat, 99)),
+                            "steps": [x[5] for x in info][:8]}}
+    # the plain per-grid FP64 loop on the same requests, FIFO
+    service, errs = [], []
+    for rid, r in zip(ids, runs):
+        p64 = hydro_amd.Params(**{**r.params.__dict__, "precision": hydro_amd.FP64})
+        t0 = time.perf_counter()
+        with hydro_amd.Grid(p64) as g:
+            g.init_blast(r.bx, r.by, r.e)
+            g.run(10**6, log=False)
+            u64 = g.download()
+        service.append(time.perf_counter() - t0)
+        um = srv.result(rid)
+        srv.release(rid)
+        e = []
+        for v in (0, 3):
+            a, b = um[v, 2:-2, 2:-2], u64[v, 2:-2, 2:-2]
+            e.append((float(np.abs(a - b).sum() / np.abs(b).sum()),
+                      float(np.abs(a - b).max() / np.abs(b).max())))
+        errs.append(e)
+        del um, u64
+    srv.close()
+    done, lat_plain = 0.0, []
+    for r, sv in zip(runs, service):  # FIFO queue of one server
+        done = max(done, r.arrival) + sv
+        lat_plain.append(done - r.arrival)
+    errs = np.array(errs)
+    res["plain_fp64_per_grid"] = {"solves_per_s": n / done, "seconds": done,
+                                  "p50_s": float(np.percentile(lat_plain, 50)),
+                                  "p99_s": float(np.percentile(lat_plain, 99))}
+    res["speedup_solves_per_s"] = res["served_mixed"]["solves_per_s"] / (n / done)
+    res["mixed_error_vs_fp64"] = {"density_l1_max": float(errs[:, 0, 0].max()),
+                                  "density_linf_max": float(errs[:, 0, 1].max()),
+                                  "energy_l1_max": float(errs[:, 1, 0].max()),
+                                  "energy_linf_max": float(errs[:, 1, 1].max())}
+    return res
+
+
 def cpu_threads():
     n = os.cpu_count() or 1
     # one GPU's share of the box's host cores
@@ -111,16 +212,13 @@ def cpu_baseline(nx, ny, nsteps=8):
         mcs = float(avg.group(1)) if avg else (sum(per_step[5:]) / max(1, len(per_step[5:])))
         return {"value": mcs * 1e6, "unit": "cell-updates/s", "cores": threads,
                 "kind": "reference", "sample": sample + " (HydroC99_2Domp, gcc -O3 -fopenmp)"}
-    import numpy as np
     import oracle
     p = oracle.Params(nx=nx, ny=ny, dx=1.0 / nx, courant_factor=0.8, tend=1e30, testcase=0)
     port = oracle.Port()
     u = port.init(p)
-    port.run(p, u, 2)
     t0 = time.perf_counter()
-    port.run(p, u, 2, state=None)
+    port.run(p, u, 2)
     dt = time.perf_counter() - t0
-    del np
     return {"value": nx * ny * 2 / dt, "unit": "cell-updates/s", "cores": threads,
             "kind": "port", "sample": f"{nx}x{ny} centred blast, 2 steps (oracle restatement)"}
 
@@ -136,6 +234,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--testcase", type=int, default=0)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--batch-n", type=int, default=512)
+    ap.add_argument("--serve-n", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); the "
@@ -143,7 +247,8 @@ def main():
         sys.exit(2)
     ny = a.ny or a.nx
     prec = 0 if a.precision == "fp64" else 1
-    r = gpu_bench(a.nx, ny, a.steps, a.warmup, prec, autotune=not a.no_autotune)
+    r = gpu_bench(a.nx, ny, a.steps, a.warmup, prec, autotune=not a.no_autotune,
+                  testcase=a.testcase)
     line = {
         "metric": METRIC,
         "value": r["value"],
@@ -157,14 +262,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if prec == 0 else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"{a.nx}x{ny} centred blast, split Godunov/MUSCL, "
+        "config": {"workload": f"{a.nx}x{ny} {'centred' if a.testcase == 0 else 'corner'} "
+                               "blast, split Godunov/MUSCL, "
                                f"{'FP64' if prec == 0 else 'mixed FP32/FP64'}, "
                                "CFL 0.8, reflective walls",
-                   "nx": a.nx, "ny": ny, "testcase": 0, "steps_timed": a.steps,
+                   "nx": a.nx, "ny": ny, "testcase": a.testcase, "steps_timed": a.steps,
                    "first_timed_step": a.warmup, "launch_shape": r["tune"]},
         "roofline": r["roofline"],
         "kernels": r["kernels"],
     }
+    if not a.no_batched:
+        line["batched"] = batched_bench(a.batch_n)
+    if not a.no_serving:
+        line["serving"] = serving_bench(a.serve_n, a.serve_rate)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.nx, ny, a.cpu_steps)
     print(json.dumps(line), flush=True)

@@ -175,6 +175,23 @@ def lib() -> C.CDLL:
         getattr(L, name).restype = C.c_longlong
     L.hydro_amd_batch_bytes_needed.argtypes = [B, C.c_int, C.c_int]
     L.hydro_amd_batch_bytes_used.argtypes = [B]
+    S = C.c_void_p
+    L.hydro_amd_now.restype = C.c_double
+    L.hydro_amd_server_create.argtypes = [C.c_int, C.c_longlong, C.c_int, C.c_int]
+    L.hydro_amd_server_create.restype = S
+    L.hydro_amd_server_destroy.argtypes = [S]
+    L.hydro_amd_server_destroy.restype = None
+    for name, args in [("hydro_amd_server_submit", [S, P, C.c_int, C.c_int, C.c_double]),
+                       ("hydro_amd_server_pump", [S]),
+                       ("hydro_amd_server_counts", [S, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                    C.c_void_p]),
+                       ("hydro_amd_server_info", [S, C.c_int, C.c_void_p, C.c_void_p,
+                                                  C.c_void_p]),
+                       ("hydro_amd_server_result", [S, C.c_int, _dp]),
+                       ("hydro_amd_server_release", [S, C.c_int])]:
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
     L.process_args.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(HydroParam)]
     L.process_args.restype = None
     L.vtkfile.argtypes = [C.c_int, HydroParam, C.POINTER(HydroVar)]
@@ -397,3 +414,73 @@ class Batch:
         best = (C.c_int * 3)()
         _check(lib().hydro_amd_batch_autotune(self._h, reps, C.cast(best, C.c_void_p)))
         return dict(zip(("waves", "xiter", "yseg"), list(best)))
+
+
+def now() -> float:
+    """The library's clock (CLOCK_MONOTONIC seconds), used for request times."""
+    return float(lib().hydro_amd_now())
+
+
+class Server:
+    """Continuous-batching request loop (hydro_amd_server_*)."""
+
+    QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
+
+    def __init__(self, precision: int = MIXED, budget_bytes: int = 200 * 10**9,
+                 chunk_steps: int = 20, keep_results: bool = False):
+        self._h = lib().hydro_amd_server_create(precision, int(budget_bytes), chunk_steps,
+                                                int(keep_results))
+        if not self._h:
+            raise HydroError(last_error())
+        self.params = {}
+
+    def close(self) -> None:
+        if self._h:
+            lib().hydro_amd_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, p: Params, bx: int, by: int, e: float) -> int:
+        r = _check(lib().hydro_amd_server_submit(self._h, C.byref(p.c()), bx, by, e))
+        self.params[r] = p
+        return r
+
+    def pump(self) -> int:
+        return _check(lib().hydro_amd_server_pump(self._h))
+
+    def counts(self):
+        v = [C.c_int() for _ in range(4)]
+        _check(lib().hydro_amd_server_counts(self._h, *[C.cast(C.byref(x), C.c_void_p)
+                                                         for x in v]))
+        return dict(zip(("queued", "running", "done", "failed"), [x.value for x in v]))
+
+    def info(self, r: int):
+        """(state, submitted, admitted, finished, t, nstep)."""
+        times = (C.c_double * 3)()
+        t = C.c_double()
+        n = C.c_int()
+        st = lib().hydro_amd_server_info(self._h, r, C.cast(times, C.c_void_p),
+                                         C.cast(C.byref(t), C.c_void_p),
+                                         C.cast(C.byref(n), C.c_void_p))
+        if st < 0:
+            raise HydroError(last_error())
+        return st, times[0], times[1], times[2], t.value, n.value
+
+    def result(self, r: int) -> np.ndarray:
+        u = np.empty(self.params[r].shape())
+        _check(lib().hydro_amd_server_result(self._h, r, u))
+        return u
+
+    def release(self, r: int) -> None:
+        _check(lib().hydro_amd_server_release(self._h, r))

@@ -563,29 +563,47 @@ static int run_finish(hydro_amd_grid* g, int* steps_done) {
   return 0;
 }
 
+// Steps queued between host checks of the device loop: a run stopped by tend
+// is noticed within one chunk, so no more than this many no-op steps are
+// ever queued past its end.
+static constexpr int kRunChunk = 256;
+
 extern "C" int hydro_amd_grid_run(hydro_amd_grid* g, int max_steps, double* dt_log) {
   if (!g) return fail("null grid");
   if (max_steps < 0) return fail("max_steps < 0");
-  int cap = dt_log ? max_steps : 0;
-  if (cap > g->dlog_cap) {
-    if (g->dlog) (void) hipFree(g->dlog);
-    g->dlog = nullptr;
-    g->dlog_cap = 0;
-    HIPCHK(hipMalloc(&g->dlog, (size_t) cap * sizeof(double)));
-    g->dlog_cap = cap;
+  long left = max_steps;
+  if (left > (long) g->p.nstepmax - g->nstep) left = (long) g->p.nstepmax - g->nstep;
+  if (left < 0) left = 0;
+  int total = 0;
+  while (left > 0) {
+    const int chunk = (int) (left < kRunChunk ? left : kRunChunk);
+    const int cap = dt_log ? chunk : 0;
+    if (cap > g->dlog_cap) {
+      if (g->dlog) (void) hipFree(g->dlog);
+      g->dlog = nullptr;
+      g->dlog_cap = 0;
+      HIPCHK(hipMalloc(&g->dlog, (size_t) kRunChunk * sizeof(double)));
+      g->dlog_cap = kRunChunk;
+    }
+    if (run_enqueue(g, chunk, cap)) return -1;
+    int done = 0;
+    if (run_finish(g, &done)) return -1;
+    if (dt_log && done > 0)
+      HIPCHK(hipMemcpy(dt_log + total, g->dlog, (size_t) done * sizeof(double),
+                       hipMemcpyDeviceToHost));
+    total += done;
+    left -= chunk;
+    if (done < chunk) break;  // the device loop ended (t >= tend)
   }
-  if (run_enqueue(g, max_steps, cap)) return -1;
-  int done = 0;
-  if (run_finish(g, &done)) return -1;
-  if (dt_log && done > 0) {
-    HIPCHK(hipMemcpy(dt_log, g->dlog, (size_t) done * sizeof(double), hipMemcpyDeviceToHost));
-  }
-  return done;
+  return total;
 }
 
 extern "C" int hydro_amd_grid_run_async(hydro_amd_grid* g, int max_steps) {
   if (!g) return fail("null grid");
-  if (run_enqueue(g, max_steps, 0)) return -1;
+  if (max_steps < 0) return fail("max_steps < 0");
+  long n = max_steps;
+  if (n > (long) g->p.nstepmax - g->nstep) n = (long) g->p.nstepmax - g->nstep;
+  if (run_enqueue(g, n > 0 ? (int) n : 0, 0)) return -1;
   return 0;
 }
 
@@ -768,9 +786,14 @@ __attribute__((weak)) long nbFLOPS;
 }
 
 namespace {
+// The GPU grid behind a surface uold is created on first GPU use
+// (allocate_work_space, compute_deltat, hydro_godunov, ...), from the host
+// state hydro_init built; until then the host array is the state.
 struct SurfaceCtx {
   hydro_amd_grid* g = nullptr;
   size_t n = 0;  // elements of uold
+  hydro_amd_params p;
+  double* host = nullptr;
 };
 std::map<const double*, SurfaceCtx> g_surface;
 const double* g_last_uold = nullptr;
@@ -781,12 +804,19 @@ const double* g_last_uold = nullptr;
   abort();
 }
 
+void ensure_grid(SurfaceCtx& c, const char* where) {
+  if (c.g) return;
+  c.g = hydro_amd_grid_create(&c.p);
+  if (!c.g || hydro_amd_grid_upload(c.g, c.host)) die(where);
+}
+
 SurfaceCtx& ctx_of(const hydrovar_t* Hv, const char* where) {
   auto it = g_surface.find(Hv ? Hv->uold : nullptr);
   if (it == g_surface.end()) {
     fail("uold %p was not created by hydro_init", Hv ? (void*) Hv->uold : nullptr);
     die(where);
   }
+  ensure_grid(it->second, where);
   return it->second;
 }
 
@@ -796,6 +826,7 @@ SurfaceCtx& last_ctx(const char* where) {
     fail("no grid: call hydro_init first");
     die(where);
   }
+  ensure_grid(it->second, where);
   return it->second;
 }
 }  // namespace
@@ -842,12 +873,13 @@ extern "C" void hydro_init(hydroparam_t* H, hydrovar_t* Hv) {
     abort();
   }
   fflush(stdout);
-  hydro_amd_params p;
-  hydro_amd_params_from_surface(H, &p);
-  hydro_amd_grid* g = hydro_amd_grid_create(&p);
-  if (!g || hydro_amd_grid_upload(g, u)) die("hydro_init");
+  SurfaceCtx c;
+  hydro_amd_params_from_surface(H, &c.p);
+  if (validate(&c.p)) die("hydro_init");
+  c.n = n;
+  c.host = u;
   Hv->uold = u;
-  g_surface[u] = SurfaceCtx{g, n};
+  g_surface[u] = c;
   g_last_uold = u;
 }
 
@@ -926,16 +958,20 @@ extern "C" void make_boundary(int idim, const hydroparam_t H, hydrovar_t* Hv) {
 extern "C" int hydro_amd_sync_host(hydrovar_t* Hv) {
   auto it = g_surface.find(Hv ? Hv->uold : nullptr);
   if (it == g_surface.end()) return fail("uold was not created by hydro_init");
+  if (!it->second.g) return 0;  // no GPU state yet: the host array is current
   return hydro_amd_grid_download(it->second.g, Hv->uold);
 }
 
 extern "C" hydro_amd_grid* hydro_amd_surface_grid(hydrovar_t* Hv) {
   auto it = g_surface.find(Hv ? Hv->uold : nullptr);
-  return it == g_surface.end() ? nullptr : it->second.g;
+  if (it == g_surface.end()) return nullptr;
+  ensure_grid(it->second, "hydro_amd_surface_grid");
+  return it->second.g;
 }
 
 extern "C" int hydro_amd_sync_device(hydrovar_t* Hv) {
   auto it = g_surface.find(Hv ? Hv->uold : nullptr);
   if (it == g_surface.end()) return fail("uold was not created by hydro_init");
+  if (!it->second.g) return 0;  // the grid will be built from the host array
   return hydro_amd_grid_upload(it->second.g, Hv->uold);
 }

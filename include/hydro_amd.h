@@ -67,11 +67,13 @@ int hydro_amd_grid_compute_deltat(hydro_amd_grid *g, double *dt);
 /* The time loop of HydroC99_2DMpi/Src/main.c:244-298 run on the GPU: up to
  * max_steps steps while t < tend and nstep < nstepmax, dt recomputed on even
  * steps and halved at step 0, sweep order alternating.  Steps are queued as a
- * replayed HIP graph; dt_log (host, may be NULL, max_steps entries) receives
- * the dt of each step taken.  Returns the number of steps taken (>= 0). */
+ * replayed HIP graph, 256 at a time, and the host stops queueing when the
+ * device loop has ended; dt_log (host, may be NULL, max_steps entries)
+ * receives the dt of each step taken.  Returns the number of steps taken. */
 int hydro_amd_grid_run(hydro_amd_grid *g, int max_steps, double *dt_log);
-/* Enqueue the same loop without waiting (steps fixed at enqueue time); pair
- * with hydro_amd_grid_sync. */
+/* Enqueue min(max_steps, nstepmax - nstep) steps of the same loop without
+ * waiting (steps past the end of the loop are no-ops); pair with
+ * hydro_amd_grid_sync. */
 int hydro_amd_grid_run_async(hydro_amd_grid *g, int max_steps);
 int hydro_amd_grid_sync(hydro_amd_grid *g);
 int hydro_amd_grid_state(hydro_amd_grid *g, double *t, int *nstep, double *dt);
@@ -135,6 +137,35 @@ int hydro_amd_batch_query(hydro_amd_batch *b, int *ids, double *t, int *nstep, d
  * unchanged).  best[3] receives waves, xiter, yseg. */
 int hydro_amd_batch_set_tune(hydro_amd_batch *b, int waves, int xiter, int yseg);
 int hydro_amd_batch_autotune(hydro_amd_batch *b, int reps, int *best);
+
+/* ---- serving: continuous batching of independent runs --------------------
+ * Requests queue in arrival order.  Each pump admits queued requests while
+ * they fit the HBM budget (FIFO; a request larger than the whole budget
+ * fails), advances every running request by chunk_steps steps in one batched
+ * device run -- requests admitted now join those already on the GPU -- and
+ * retires the runs whose time loop has ended (t >= tend or nstep >=
+ * nstepmax), copying results out when keep_results and returning their
+ * memory.  A run whose dt stops being positive and finite is failed alone.
+ * Timestamps use hydro_amd_now() (CLOCK_MONOTONIC seconds). */
+typedef struct hydro_amd_server hydro_amd_server;
+double hydro_amd_now(void);
+hydro_amd_server *hydro_amd_server_create(int precision, long long budget_bytes, int chunk_steps,
+                                          int keep_results);
+void hydro_amd_server_destroy(hydro_amd_server *s);
+hydro_amd_batch *hydro_amd_server_batch(hydro_amd_server *s);
+/* Queue a run of p (its tend / nstepmax end it) from the blast initial state
+ * (energy e at interior cell bx, by); returns the request id. */
+int hydro_amd_server_submit(hydro_amd_server *s, const hydro_amd_params *p, int bx, int by,
+                            double e);
+/* One round; returns requests still queued or running (0 = idle), -1 error. */
+int hydro_amd_server_pump(hydro_amd_server *s);
+int hydro_amd_server_counts(hydro_amd_server *s, int *queued, int *running, int *done,
+                            int *failed);
+/* times[3] = submitted, admitted, finished; returns the state: 0 queued,
+ * 1 running, 2 done, 3 failed (or -1 for an unknown id). */
+int hydro_amd_server_info(hydro_amd_server *s, int req, double *times, double *t_sim, int *nstep);
+int hydro_amd_server_result(hydro_amd_server *s, int req, double *uold);
+int hydro_amd_server_release(hydro_amd_server *s, int req);
 
 /* Host mirror of the reference surface (hydro_surface.h): refresh Hv->uold
  * from the GPU, or push host edits of Hv->uold to the GPU. */

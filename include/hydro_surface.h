@@ -14,10 +14,11 @@
  *    caller that edits Hv->uold on the host must call hydro_amd_sync_device()
  *    (declared in hydro_amd.h) before the next step.  The reference driver
  *    (main.c) never touches uold between steps, so it needs neither.
- *  - hydrowork_t / hydrovarwork_t are opaque: the GPU path keeps no host work
- *    arrays, their pointers are left NULL.  allocate_work_space() allocates
- *    the GPU ping-pong buffer and compute_deltat_init_mem() the GPU CFL
- *    reduction slots of the most recently initialised grid.
+ *  - hydro_init() builds the initial state on the host only; the GPU state
+ *    (state + ping-pong buffer + CFL slots) is created from it on the first
+ *    GPU-side call (allocate_work_space, compute_deltat, hydro_godunov, ...).
+ *    hydrowork_t / hydrovarwork_t are opaque: the GPU path keeps no host work
+ *    arrays, their pointers are left NULL.
  *  - Failures print a message and abort(), as the reference's asserts do.
  *  - Not reentrant, one thread at a time (as the reference).
  */
@@ -115,9 +116,9 @@ extern long nbFLOPS;
  * Unknown keys are ignored, as in the reference. */
 void process_args(int argc, char **argv, hydroparam_t *H);
 
-/* HydroC99_2DMpi/Src/hydro_funcs.h:41: grid dims, host uold + GPU state,
- * initial condition of hydro_funcs.c:79-146 (testcases 0 centre, 1 corner,
- * 2 Sod column; others abort as in the reference).  Uploads the state. */
+/* HydroC99_2DMpi/Src/hydro_funcs.h:41: grid dims, host uold, initial
+ * condition of hydro_funcs.c:79-146 (testcases 0 centre, 1 corner, 2 Sod
+ * column; others abort as in the reference). */
 void hydro_init(hydroparam_t *H, hydrovar_t *Hv);
 /* HydroC99_2DMpi/Src/hydro_funcs.h:42 */
 void hydro_finish(const hydroparam_t H, hydrovar_t *Hv);
