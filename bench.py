@@ -223,6 +223,21 @@ def cpu_baseline(nx, ny, nsteps=8):
             "kind": "port", "sample": f"{nx}x{ny} centred blast, 2 steps (oracle restatement)"}
 
 
+def pmc_workload(nx, precision, steps):
+    """A short fixed workload for counter runs (rocprofv3 --pmc): one plain
+    copy kernel of 1 GiB (the byte-counter calibration), then `steps` steps of
+    the bench grid without the autotuner."""
+    import hydro_amd
+    gbs = hydro_amd.copy_probe(1 << 30, 1)
+    p = hydro_amd.Params(nx=nx, ny=nx, dx=1.0 / nx, courant_factor=0.8, tend=1e30,
+                         precision=precision)
+    with hydro_amd.Grid(p) as g:
+        g.init(0)
+        g.run(steps, log=False)
+    print(json.dumps({"pmc_workload": True, "copy_bytes": 1 << 30, "copy_GBps": gbs,
+                      "nx": nx, "steps": steps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,6 +255,8 @@ def main():
     ap.add_argument("--batch-n", type=int, default=512)
     ap.add_argument("--serve-n", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="short fixed run for rocprofv3 counter passes")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); the "
@@ -247,6 +264,9 @@ def main():
         sys.exit(2)
     ny = a.ny or a.nx
     prec = 0 if a.precision == "fp64" else 1
+    if a.pmc_workload:
+        pmc_workload(a.nx, prec, a.steps)
+        return
     r = gpu_bench(a.nx, ny, a.steps, a.warmup, prec, autotune=not a.no_autotune,
                   testcase=a.testcase)
     line = {

@@ -177,6 +177,8 @@ def lib() -> C.CDLL:
     L.hydro_amd_batch_bytes_used.argtypes = [B]
     S = C.c_void_p
     L.hydro_amd_now.restype = C.c_double
+    L.hydro_amd_copy_probe.argtypes = [C.c_longlong, C.c_int]
+    L.hydro_amd_copy_probe.restype = C.c_double
     L.hydro_amd_server_create.argtypes = [C.c_int, C.c_longlong, C.c_int, C.c_int]
     L.hydro_amd_server_create.restype = S
     L.hydro_amd_server_destroy.argtypes = [S]
@@ -394,7 +396,8 @@ class Batch:
         _check(lib().hydro_amd_batch_sync(self._h))
 
     def query(self):
-        """{id: (t, nstep, dt, more_steps)} for every live instance."""
+        """{id: (t, nstep, dt, state)} for every live instance; state 1 = more
+        steps to run, 0 = ended, -1 = stopped at a non-positive/non-finite dt."""
         cap = max(len(self.params), 1)
         ids = np.zeros(cap, np.int32)
         t = np.zeros(cap)
@@ -404,7 +407,7 @@ class Batch:
         ptr = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
         k = _check(lib().hydro_amd_batch_query(self._h, ptr(ids), ptr(t), ptr(n), ptr(dt),
                                                ptr(act), cap))
-        return {int(ids[j]): (float(t[j]), int(n[j]), float(dt[j]), bool(act[j]))
+        return {int(ids[j]): (float(t[j]), int(n[j]), float(dt[j]), int(act[j]))
                 for j in range(min(k, cap))}
 
     def set_tune(self, waves: int, xiter: int, yseg: int) -> None:
@@ -414,6 +417,14 @@ class Batch:
         best = (C.c_int * 3)()
         _check(lib().hydro_amd_batch_autotune(self._h, reps, C.cast(best, C.c_void_p)))
         return dict(zip(("waves", "xiter", "yseg"), list(best)))
+
+
+def copy_probe(nbytes: int, reps: int = 10) -> float:
+    """GB/s (read + written) of a plain copy kernel over nbytes."""
+    v = lib().hydro_amd_copy_probe(int(nbytes), reps)
+    if v < 0:
+        raise HydroError(last_error())
+    return float(v)
 
 
 def now() -> float:

@@ -55,18 +55,36 @@ struct hydro_amd_batch {
   RunCtl* ctl = nullptr;         // kMaxInst run controls
   uint64_t* cour = nullptr;      // kMaxInst * 2 * kInstSlots
   void* idev = nullptr;          // InstDev<S>[kMaxInst]
-  BTask* tasks = nullptr;        // P, Q, Pb, Qb
+  // task lists, in this order: Px Py (phase P: x of even-step instances, y of
+  // odd ones), Qx Qy (phase Q: the other sweep of each), PB QB (ghost fills)
+  BTask* tasks = nullptr;
   int task_cap = 0;
   int* ids = nullptr;            // live ids (device)
   int* cids = nullptr;           // ids needing a CFL pass (device)
-  int nP = 0, nQ = 0, nB = 0, nids = 0;
+  int n[6] = {0, 0, 0, 0, 0, 0}; // Px Py Qx Qy PB QB
+  int off[6] = {0, 0, 0, 0, 0, 0};
+  int nids = 0;
+  int maxid = -1;                // highest live id
   bool dirty = true;
   bool flip = false;             // next step runs Q before P
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   BatchTune tune = {4, 4, 32};
-  int nblocks = 1024;
+  int nblk_x = 1024, nblk_y = 1024, cus = 256;
   std::vector<int> live_ids;
+  RunCtl* hctl = nullptr;        // pinned host copy of the run controls
+  void* hidev = nullptr;         // pinned host copy of the instance array
 };
+
+static void size_grids(hydro_amd_batch* b) {
+  const int w = b->tune.waves;
+  if (b->prec == HYDRO_AMD_FP64) {
+    b->nblk_x = b->cus * batch_sweep_occupancy<double>(1, w);
+    b->nblk_y = b->cus * batch_sweep_occupancy<double>(2, w);
+  } else {
+    b->nblk_x = b->cus * batch_sweep_occupancy<float>(1, w);
+    b->nblk_y = b->cus * batch_sweep_occupancy<float>(2, w);
+  }
+}
 
 static void drop_graphs(hydro_amd_batch* b) {
   for (auto& g : b->gexec)
@@ -86,6 +104,8 @@ extern "C" void hydro_amd_batch_destroy(hydro_amd_batch* b) {
   for (void* p : {(void*) b->ctl, (void*) b->cour, b->idev, (void*) b->tasks, (void*) b->ids,
                   (void*) b->cids})
     if (p) (void) hipFree(p);
+  if (b->hctl) (void) hipHostFree(b->hctl);
+  if (b->hidev) (void) hipHostFree(b->hidev);
   if (b->st) (void) hipStreamDestroy(b->st);
   delete b;
 }
@@ -106,7 +126,9 @@ extern "C" hydro_amd_batch* hydro_amd_batch_create(int precision, long long budg
             hipMalloc(&b->cour, sizeof(uint64_t) * 2 * kInstSlots * kMaxInst) == hipSuccess &&
             hipMalloc(&b->idev, isz * kMaxInst) == hipSuccess &&
             hipMalloc(&b->ids, sizeof(int) * kMaxInst) == hipSuccess &&
-            hipMalloc(&b->cids, sizeof(int) * kMaxInst) == hipSuccess;
+            hipMalloc(&b->cids, sizeof(int) * kMaxInst) == hipSuccess &&
+            hipHostMalloc((void**) &b->hctl, sizeof(RunCtl) * kMaxInst) == hipSuccess &&
+            hipHostMalloc(&b->hidev, isz * kMaxInst) == hipSuccess;
   if (!ok) {
     hydro_amd_set_error("batch allocation failed");
     hydro_amd_batch_destroy(b);
@@ -120,12 +142,9 @@ extern "C" hydro_amd_batch* hydro_amd_batch_create(int precision, long long budg
     uint64_t thr = ~0ULL;
     (void) hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
   }
-  int cus = 256;
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
-  const int occ = precision == HYDRO_AMD_FP64 ? batch_sweep_occupancy<double>(b->tune.waves)
-                                              : batch_sweep_occupancy<float>(b->tune.waves);
-  b->nblocks = cus * occ;
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) b->cus = prop.multiProcessorCount;
+  size_grids(b);
   if (hipStreamSynchronize(b->st) != hipSuccess) {
     hydro_amd_set_error("batch setup failed");
     hydro_amd_batch_destroy(b);
@@ -281,6 +300,21 @@ extern "C" int hydro_amd_batch_download(hydro_amd_batch* b, int id, double* u) {
   return 0;
 }
 
+// Copies an instance's state as stored (element size 8 or 4), layout
+// [4][ny+4][nx+4]; for the serving loop (no conversion on its critical path).
+int hydro_amd_batch_download_raw(hydro_amd_batch* b, int id, void* dst, int* elem_size) {
+  Inst* in = live(b, id);
+  if (!in || !dst) return in ? hydro_amd_set_error("null argument") : -1;
+  const long nxt = in->p.nx + 4, rows = 4L * (in->p.ny + 4);
+  const size_t w = b->esz;
+  const void* src = (const char*) in->mem[0] + kOrigin * w;
+  BCHK(hipMemcpy2DAsync(dst, nxt * w, src, in->g.pitch * w, nxt * w, rows,
+                        hipMemcpyDeviceToHost, b->st));
+  BCHK(hipStreamSynchronize(b->st));
+  if (elem_size) *elem_size = (int) w;
+  return 0;
+}
+
 extern "C" int hydro_amd_batch_upload(hydro_amd_batch* b, int id, const double* u) {
   Inst* in = live(b, id);
   if (!in || !u) return in ? hydro_amd_set_error("null argument") : -1;
@@ -347,28 +381,42 @@ static void tasks_of(const hydro_amd_batch* b, int id, int kind, std::vector<BTa
   }
 }
 
+// Reads the run controls of ids [0, maxid] into the pinned host copy.
+static int read_ctl(hydro_amd_batch* b) {
+  if (b->maxid < 0) return 0;
+  BCHK(hipMemcpyAsync(b->hctl, b->ctl, sizeof(RunCtl) * (b->maxid + 1), hipMemcpyDeviceToHost,
+                      b->st));
+  BCHK(hipStreamSynchronize(b->st));
+  return 0;
+}
+
 // Rebuilds the device instance array and the phase tables from the current
 // live set and their step parities (the stream must be idle).
 static int rebuild(hydro_amd_batch* b) {
   drop_graphs(b);
-  std::vector<RunCtl> ctl(kMaxInst);
-  BCHK(hipMemcpy(ctl.data(), b->ctl, sizeof(RunCtl) * kMaxInst, hipMemcpyDeviceToHost));
   b->live_ids.clear();
+  b->maxid = -1;
   for (int k = 0; k < kMaxInst; k++)
-    if (b->inst[k].live) b->live_ids.push_back(k);
-  std::vector<BTask> P, Q, PB, QB;
+    if (b->inst[k].live) {
+      b->live_ids.push_back(k);
+      b->maxid = k;
+    }
+  if (read_ctl(b)) return -1;
+  std::vector<BTask> L[6];
   for (int id : b->live_ids) {
-    const bool even = (ctl[id].nstep % 2) == 0;
-    tasks_of(b, id, even ? 1 : 2, P);
-    tasks_of(b, id, even ? 2 : 1, Q);
-    PB.push_back(BTask{id, even ? 1 : 2, 0, 0});
-    QB.push_back(BTask{id, even ? 2 : 1, 0, 0});
+    const bool even = (b->hctl[id].nstep % 2) == 0;
+    tasks_of(b, id, 1, even ? L[0] : L[2]);
+    tasks_of(b, id, 2, even ? L[3] : L[1]);
+    L[4].push_back(BTask{id, even ? 1 : 2, 0, 0});
+    L[5].push_back(BTask{id, even ? 2 : 1, 0, 0});
   }
-  b->nP = (int) P.size();
-  b->nQ = (int) Q.size();
-  b->nB = (int) PB.size();
+  int need = 0;
+  for (int k = 0; k < 6; k++) {
+    b->off[k] = need;
+    b->n[k] = (int) L[k].size();
+    need += b->n[k];
+  }
   b->nids = (int) b->live_ids.size();
-  const int need = b->nP + b->nQ + 2 * b->nB;
   if (need > b->task_cap) {
     if (b->tasks) BCHK(hipFree(b->tasks));
     b->tasks = nullptr;
@@ -379,43 +427,43 @@ static int rebuild(hydro_amd_batch* b) {
   }
   std::vector<BTask> all;
   all.reserve(need);
-  all.insert(all.end(), P.begin(), P.end());
-  all.insert(all.end(), Q.begin(), Q.end());
-  all.insert(all.end(), PB.begin(), PB.end());
-  all.insert(all.end(), QB.begin(), QB.end());
+  for (auto& l : L) all.insert(all.end(), l.begin(), l.end());
   if (!all.empty())
     BCHK(hipMemcpy(b->tasks, all.data(), sizeof(BTask) * all.size(), hipMemcpyHostToDevice));
   if (b->nids)
     BCHK(hipMemcpy(b->ids, b->live_ids.data(), sizeof(int) * b->nids, hipMemcpyHostToDevice));
-  if (b->prec == HYDRO_AMD_FP64) {
-    std::vector<InstDev<double>> v(kMaxInst);
-    for (int id : b->live_ids) v[id] = inst_dev<double>(b, id);
-    BCHK(hipMemcpy(b->idev, v.data(), sizeof(v[0]) * kMaxInst, hipMemcpyHostToDevice));
-  } else {
-    std::vector<InstDev<float>> v(kMaxInst);
-    for (int id : b->live_ids) v[id] = inst_dev<float>(b, id);
-    BCHK(hipMemcpy(b->idev, v.data(), sizeof(v[0]) * kMaxInst, hipMemcpyHostToDevice));
+  if (b->maxid >= 0) {
+    if (b->prec == HYDRO_AMD_FP64) {
+      auto* v = (InstDev<double>*) b->hidev;
+      for (int id : b->live_ids) v[id] = inst_dev<double>(b, id);
+      BCHK(hipMemcpy(b->idev, v, sizeof(v[0]) * (b->maxid + 1), hipMemcpyHostToDevice));
+    } else {
+      auto* v = (InstDev<float>*) b->hidev;
+      for (int id : b->live_ids) v[id] = inst_dev<float>(b, id);
+      BCHK(hipMemcpy(b->idev, v, sizeof(v[0]) * (b->maxid + 1), hipMemcpyHostToDevice));
+    }
   }
   b->flip = false;
   b->dirty = false;
   return 0;
 }
 
+// One phase: its ghost fills, then its x tasks and its y tasks.
+template <typename S>
+static void enq_phase(hydro_amd_batch* b, int which, int phase) {
+  const InstDev<S>* I = (const InstDev<S>*) b->idev;
+  const int x = which == 0 ? 0 : 2, y = x + 1, bd = which == 0 ? 4 : 5;
+  launch_batch_boundary<S>(I, b->tasks + b->off[bd], b->n[bd], phase, b->nblk_y, b->st);
+  launch_batch_sweep<S>(1, I, b->tasks + b->off[x], b->n[x], phase, b->tune, b->nblk_x, b->st);
+  launch_batch_sweep<S>(2, I, b->tasks + b->off[y], b->n[y], phase, b->tune, b->nblk_y, b->st);
+}
+
 template <typename S>
 static void enq_step_t(hydro_amd_batch* b, bool flip) {
   const InstDev<S>* I = (const InstDev<S>*) b->idev;
-  const BTask* P = b->tasks;
-  const BTask* Q = b->tasks + b->nP;
-  const BTask* PB = b->tasks + b->nP + b->nQ;
-  const BTask* QB = PB + b->nB;
-  const BTask* s1 = flip ? Q : P;
-  const BTask* s2 = flip ? P : Q;
-  const int n1 = flip ? b->nQ : b->nP, n2 = flip ? b->nP : b->nQ;
   launch_batch_ctl<S>(I, b->ids, b->nids, true, b->st);
-  launch_batch_boundary<S>(I, flip ? QB : PB, b->nB, 0, b->nblocks, b->st);
-  launch_batch_sweep<S>(I, s1, n1, 0, b->tune, b->nblocks, b->st);
-  launch_batch_boundary<S>(I, flip ? PB : QB, b->nB, 1, b->nblocks, b->st);
-  launch_batch_sweep<S>(I, s2, n2, 1, b->tune, b->nblocks, b->st);
+  enq_phase<S>(b, flip ? 1 : 0, 0);
+  enq_phase<S>(b, flip ? 0 : 1, 1);
   launch_batch_ctl<S>(I, b->ids, b->nids, false, b->st);
 }
 
@@ -500,15 +548,19 @@ extern "C" int hydro_amd_batch_query(hydro_amd_batch* b, int* ids, double* t, in
                                      double* dt, int* active, int cap) {
   if (!b) return hydro_amd_set_error("null batch");
   BCHK(hipStreamSynchronize(b->st));
-  std::vector<RunCtl> ctl(kMaxInst);
-  BCHK(hipMemcpy(ctl.data(), b->ctl, sizeof(RunCtl) * kMaxInst, hipMemcpyDeviceToHost));
+  int top = -1;
+  for (int k = 0; k < kMaxInst; k++)
+    if (b->inst[k].live) top = k;
+  b->maxid = top > b->maxid ? top : b->maxid;
+  if (read_ctl(b)) return -1;
   int n = 0;
-  for (int k = 0; k < kMaxInst; k++) {
+  for (int k = 0; k <= top; k++) {
     if (!b->inst[k].live) continue;
     if (n < cap) {
-      const RunCtl& c = ctl[k];
-      // a step opens only if t < tend && nstep < nstepmax (main.c:244)
-      const int more = (c.t < c.tend) && (c.nstep < c.nstepmax);
+      const RunCtl& c = b->hctl[k];
+      // a step opens only if t < tend && nstep < nstepmax (main.c:244);
+      // -1: stopped at a dt that was not positive and finite
+      const int more = c.fault ? -1 : ((c.t < c.tend) && (c.nstep < c.nstepmax));
       if (ids) ids[n] = k;
       if (t) t[n] = c.t;
       if (nstep) nstep[n] = c.nstep;
@@ -526,13 +578,7 @@ extern "C" int hydro_amd_batch_set_tune(hydro_amd_batch* b, int waves, int xiter
     return hydro_amd_set_error("tune out of range");
   BCHK(hipStreamSynchronize(b->st));
   b->tune = BatchTune{waves, xiter, yseg};
-  int dev = 0, cus = 256;
-  hipDeviceProp_t prop;
-  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-    cus = prop.multiProcessorCount;
-  const int occ = b->prec == HYDRO_AMD_FP64 ? batch_sweep_occupancy<double>(waves)
-                                            : batch_sweep_occupancy<float>(waves);
-  b->nblocks = cus * occ;
+  size_grids(b);
   b->dirty = true;
   return 0;
 }
@@ -547,14 +593,16 @@ extern "C" int hydro_amd_batch_autotune(hydro_amd_batch* b, int reps, int* best)
   BCHK(hipStreamSynchronize(b->st));
   if (b->dirty && rebuild(b)) return -1;
   if (b->nids == 0) return hydro_amd_set_error("empty batch");
-  std::vector<RunCtl> saved(kMaxInst);
-  BCHK(hipMemcpy(saved.data(), b->ctl, sizeof(RunCtl) * kMaxInst, hipMemcpyDeviceToHost));
+  const int nc = b->maxid + 1;
+  std::vector<RunCtl> saved(nc);
+  BCHK(hipMemcpy(saved.data(), b->ctl, sizeof(RunCtl) * nc, hipMemcpyDeviceToHost));
   std::vector<RunCtl> forced = saved;
   for (int id : b->live_ids) {
     forced[id].active = 1;
+    forced[id].fault = 0;
     if (!(forced[id].dt > 0)) forced[id].dt = 1e-12;
   }
-  BCHK(hipMemcpy(b->ctl, forced.data(), sizeof(RunCtl) * kMaxInst, hipMemcpyHostToDevice));
+  BCHK(hipMemcpy(b->ctl, forced.data(), sizeof(RunCtl) * nc, hipMemcpyHostToDevice));
   const BatchTune keep = b->tune;
   BatchTune bt = keep;
   double bms = 1e30;
@@ -567,14 +615,16 @@ extern "C" int hydro_amd_batch_autotune(hydro_amd_batch* b, int reps, int* best)
         if (hydro_amd_batch_set_tune(b, w, xi, ys) || rebuild(b)) break;
         for (int r = -1; r < reps; r++) {  // r = -1 warms up
           if (r == 0) (void) hipEventRecord(e0, b->st);
-          if (b->prec == HYDRO_AMD_FP64) {
-            const auto* I = (const InstDev<double>*) b->idev;
-            launch_batch_sweep<double>(I, b->tasks, b->nP, 0, b->tune, b->nblocks, b->st);
-            launch_batch_sweep<double>(I, b->tasks + b->nP, b->nQ, 0, b->tune, b->nblocks, b->st);
-          } else {
-            const auto* I = (const InstDev<float>*) b->idev;
-            launch_batch_sweep<float>(I, b->tasks, b->nP, 0, b->tune, b->nblocks, b->st);
-            launch_batch_sweep<float>(I, b->tasks + b->nP, b->nQ, 0, b->tune, b->nblocks, b->st);
+          // every x and y list at phase 0: reads buffer 0, writes buffer 1 only
+          for (int k = 0; k < 4; k++) {
+            const int kind = (k % 2 == 0) ? 1 : 2;
+            const int nb = kind == 1 ? b->nblk_x : b->nblk_y;
+            if (b->prec == HYDRO_AMD_FP64)
+              launch_batch_sweep<double>(kind, (const InstDev<double>*) b->idev,
+                                         b->tasks + b->off[k], b->n[k], 0, b->tune, nb, b->st);
+            else
+              launch_batch_sweep<float>(kind, (const InstDev<float>*) b->idev,
+                                        b->tasks + b->off[k], b->n[k], 0, b->tune, nb, b->st);
           }
         }
         (void) hipEventRecord(e1, b->st);
@@ -587,7 +637,7 @@ extern "C" int hydro_amd_batch_autotune(hydro_amd_batch* b, int reps, int* best)
       }
   (void) hipEventDestroy(e0);
   (void) hipEventDestroy(e1);
-  BCHK(hipMemcpy(b->ctl, saved.data(), sizeof(RunCtl) * kMaxInst, hipMemcpyHostToDevice));
+  BCHK(hipMemcpy(b->ctl, saved.data(), sizeof(RunCtl) * nc, hipMemcpyHostToDevice));
   if (hydro_amd_batch_set_tune(b, bt.waves, bt.xiter, bt.yseg)) return -1;
   if (best) {
     best[0] = bt.waves;

@@ -18,9 +18,10 @@
 
 namespace hydro {
 
+// x tasks of one phase (every task's kind is 1).
 template <typename S>
-__global__ __launch_bounds__(256) void k_batch_sweep(const InstDev<S>* inst, const BTask* tasks,
-                                                     int ntask, int phase, int xiter, int yseg) {
+__global__ __launch_bounds__(256) void k_batch_sweep_x(const InstDev<S>* inst, const BTask* tasks,
+                                                       int ntask, int phase, int xiter) {
   const int wpb = blockDim.x >> 6;
   const int wave = threadIdx.x >> 6;
   for (int t = blockIdx.x; t < ntask; t += gridDim.x) {
@@ -32,36 +33,53 @@ __global__ __launch_bounds__(256) void k_batch_sweep(const InstDev<S>* inst, con
     const GridView<S> in = d.buf[phase], out = d.buf[1 - phase];
     // the second sweep of an odd step is an x sweep; it produces the CFL
     // maxima the next (even) step's dt needs
-    const bool cour = phase == 1 && tk.kind == 1;
+    const bool cour = phase == 1;
     const bool st1 = d.ph.slope_type == (S) 1;
     double cx = 0.0, cy = 0.0;
-    if (tk.kind == 1) {
-      const long ntot = (long) d.nseg * d.g.ny;
-      for (int it = 0; it < xiter; it++) {
-        const long s = (long) tk.a + (long) it * wpb + wave;
-        const bool ok = s < ntot && s < (long) tk.a + (long) wpb * xiter;
-        const long jrow = ok ? s / d.nseg : 0;
-        const int seg = ok ? (int) (s - jrow * d.nseg) : 0;
-        if (st1) {
-          if (cour) x_segment<S, true, true>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
-          else x_segment<S, true, false>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
-        } else {
-          if (cour) x_segment<S, false, true>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
-          else x_segment<S, false, false>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
-        }
-      }
-    } else {
-      const int col = tk.a * 64 + (threadIdx.x & 63);
-      const bool colok = col < d.g.nx;
-      const long i = 2 + (colok ? col : d.g.nx - 1);
-      const int j0 = 2 + (tk.b * wpb + wave) * yseg;
-      if (j0 < d.g.ny + 2) {  // wave-uniform
-        const int j1 = min(j0 + yseg, d.g.ny + 2);
-        if (st1) y_march<S, true, false>(d.g, d.ph, tr, dtdx, in, out, colok, i, j0, j1, cx, cy);
-        else y_march<S, false, false>(d.g, d.ph, tr, dtdx, in, out, colok, i, j0, j1, cx, cy);
+    const long ntot = (long) d.nseg * d.g.ny;
+    for (int it = 0; it < xiter; it++) {
+      const long s = (long) tk.a + (long) it * wpb + wave;
+      const bool ok = s < ntot;
+      const long jrow = ok ? s / d.nseg : 0;
+      const int seg = ok ? (int) (s - jrow * d.nseg) : 0;
+      if (st1) {
+        if (cour) x_segment<S, true, true>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
+        else x_segment<S, true, false>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
+      } else {
+        if (cour) x_segment<S, false, true>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
+        else x_segment<S, false, false>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
       }
     }
     if (cour) commit_courant(cx, cy, d.cour, kInstSlots, t % kInstSlots);
+  }
+}
+
+// y tasks of one phase (every task's kind is 2); y sweeps never end a step
+// whose successor needs dt, so they fold no CFL maxima.
+template <typename S>
+__global__ __launch_bounds__(256) void k_batch_sweep_y(const InstDev<S>* inst, const BTask* tasks,
+                                                       int ntask, int phase, int yseg) {
+  const int wpb = blockDim.x >> 6;
+  const int wave = threadIdx.x >> 6;
+  for (int t = blockIdx.x; t < ntask; t += gridDim.x) {
+    const BTask tk = tasks[t];
+    const InstDev<S>& d = inst[tk.inst];
+    if (!d.ctl->active) continue;
+    const double dtdx = d.ctl->dt / d.dx;
+    const TraceK<S> tr = make_tracek<S>((S) dtdx, d.ph.scheme);
+    const GridView<S> in = d.buf[phase], out = d.buf[1 - phase];
+    const int col = tk.a * 64 + (threadIdx.x & 63);
+    const bool colok = col < d.g.nx;
+    const long i = 2 + (colok ? col : d.g.nx - 1);
+    const int j0 = 2 + (tk.b * wpb + wave) * yseg;
+    double cx = 0.0, cy = 0.0;
+    if (j0 < d.g.ny + 2) {  // wave-uniform
+      const int j1 = min(j0 + yseg, d.g.ny + 2);
+      if (d.ph.slope_type == (S) 1)
+        y_march<S, true, false>(d.g, d.ph, tr, dtdx, in, out, colok, i, j0, j1, cx, cy);
+      else
+        y_march<S, false, false>(d.g, d.ph, tr, dtdx, in, out, colok, i, j0, j1, cx, cy);
+    }
   }
 }
 
@@ -107,7 +125,10 @@ __global__ __launch_bounds__(256) void k_batch_courant(const InstDev<S>* inst, c
   }
 }
 
-// main.c:244-260 for every listed instance: one wave per instance.
+// main.c:244-260 for every listed instance: one wave per instance.  A batch
+// also stops an instance at its first dt that is not positive and finite
+// (the reference would go on; this only differs for runs that have already
+// broken down) and flags it, so one bad run cannot stall the others.
 template <typename S>
 __global__ __launch_bounds__(64) void k_batch_ctl_begin(const InstDev<S>* inst, const int* ids,
                                                         int nids) {
@@ -116,13 +137,18 @@ __global__ __launch_bounds__(64) void k_batch_ctl_begin(const InstDev<S>* inst, 
     RunCtl* c = d.ctl;
     const double t = c->t, tend = c->tend;
     const int nstep = c->nstep, nstepmax = c->nstepmax;
-    const bool act = (t < tend) && (nstep < nstepmax);
+    bool act = (t < tend) && (nstep < nstepmax) && !c->fault;
+    bool bad = false;
     if (act && (nstep % 2) == 0) {
       double dt = wave_reduce_dt(d.cour, kInstSlots, d.cf, d.dx, d.smallc);
       if (nstep == 0) dt = dt / 2.0;
-      if (threadIdx.x == 0) c->dt = dt;
+      bad = !(dt > 0.0) || !__builtin_isfinite(dt);
+      if (threadIdx.x == 0 && !bad) c->dt = dt;
     }
-    if (threadIdx.x == 0) c->active = act ? 1 : 0;
+    if (threadIdx.x == 0) {
+      if (bad) c->fault = 1;
+      c->active = (act && !bad) ? 1 : 0;
+    }
   }
 }
 
@@ -149,6 +175,7 @@ __global__ void k_batch_set_ctl(RunCtl* c, double t, double dt, double tend, int
   c->log = nullptr;
   c->log_cap = 0;
   c->log_base = 0;
+  c->fault = 0;
 }
 
 __global__ void k_fill_u64(uint64_t* p, long n) {
@@ -173,23 +200,28 @@ void launch_zero_u64(uint64_t* p, long n, hipStream_t st) {
 }
 
 template <typename S>
-int batch_sweep_occupancy(int waves) {
+int batch_sweep_occupancy(int kind, int waves) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_sweep<S>, 64 * waves, 0) !=
-      hipSuccess)
-    return 2;
+  const hipError_t e =
+      kind == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_sweep_x<S>, 64 * waves, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_sweep_y<S>, 64 * waves, 0);
+  if (e != hipSuccess) return 2;
   return nb > 0 ? nb : 1;
 }
-template int batch_sweep_occupancy<double>(int);
-template int batch_sweep_occupancy<float>(int);
+template int batch_sweep_occupancy<double>(int, int);
+template int batch_sweep_occupancy<float>(int, int);
 
 template <typename S>
-void launch_batch_sweep(const InstDev<S>* inst, const BTask* tasks, int ntask, int phase,
-                        const BatchTune& tn, int nblocks, hipStream_t st) {
+void launch_batch_sweep(int kind, const InstDev<S>* inst, const BTask* tasks, int ntask,
+                        int phase, const BatchTune& tn, int nblocks, hipStream_t st) {
   if (ntask <= 0) return;
   const int nb = ntask < nblocks ? ntask : nblocks;
-  hipLaunchKernelGGL(k_batch_sweep<S>, dim3(nb), dim3(64 * tn.waves), 0, st, inst, tasks, ntask,
-                     phase, tn.xiter, tn.yseg);
+  if (kind == 1)
+    hipLaunchKernelGGL(k_batch_sweep_x<S>, dim3(nb), dim3(64 * tn.waves), 0, st, inst, tasks,
+                       ntask, phase, tn.xiter);
+  else
+    hipLaunchKernelGGL(k_batch_sweep_y<S>, dim3(nb), dim3(64 * tn.waves), 0, st, inst, tasks,
+                       ntask, phase, tn.yseg);
 }
 
 template <typename S>
@@ -222,7 +254,7 @@ void launch_batch_ctl(const InstDev<S>* inst, const int* ids, int nids, bool beg
 }
 
 #define INST(S)                                                                                  \
-  template void launch_batch_sweep<S>(const InstDev<S>*, const BTask*, int, int,                \
+  template void launch_batch_sweep<S>(int, const InstDev<S>*, const BTask*, int, int,           \
                                       const BatchTune&, int, hipStream_t);                        \
   template void launch_batch_boundary<S>(const InstDev<S>*, const BTask*, int, int, int,        \
                                          hipStream_t);                                           \

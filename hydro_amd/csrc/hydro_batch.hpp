@@ -33,9 +33,10 @@ struct BatchTune {
   int yseg;   // rows each wave marches in a y task
 };
 
+// kind 1: x tasks, 2: y tasks (a phase launches both, on their own tables)
 template <typename S>
-void launch_batch_sweep(const InstDev<S>* inst, const BTask* tasks, int ntask, int phase,
-                        const BatchTune& tn, int nblocks, hipStream_t st);
+void launch_batch_sweep(int kind, const InstDev<S>* inst, const BTask* tasks, int ntask,
+                        int phase, const BatchTune& tn, int nblocks, hipStream_t st);
 template <typename S>
 void launch_batch_boundary(const InstDev<S>* inst, const BTask* tasks, int ntask, int phase,
                            int nblocks, hipStream_t st);
@@ -49,6 +50,6 @@ void launch_batch_set_ctl(RunCtl* c, double t, double dt, double tend, int nstep
                           int active, hipStream_t st);
 void launch_zero_u64(uint64_t* p, long n, hipStream_t st);
 template <typename S>
-int batch_sweep_occupancy(int waves);  // resident workgroups per CU
+int batch_sweep_occupancy(int kind, int waves);  // resident workgroups per CU
 
 }  // namespace hydro

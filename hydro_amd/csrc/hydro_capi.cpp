@@ -774,6 +774,35 @@ extern "C" int hydro_amd_grid_set_tune(hydro_amd_grid* g, int yseg, int ywaves, 
   return 0;
 }
 
+// ---- measurement support -------------------------------------------------------------
+
+extern "C" double hydro_amd_copy_probe(long long bytes, int reps) {
+  if (bytes < 8 || reps < 1) return fail("bad copy probe size");
+  const long n = (long) (bytes / 8);
+  double *a = nullptr, *b = nullptr;
+  hipEvent_t e0, e1;
+  if (hipMalloc(&a, n * 8) != hipSuccess || hipMalloc(&b, n * 8) != hipSuccess) {
+    if (a) (void) hipFree(a);
+    return fail("copy probe allocation failed");
+  }
+  (void) hipMemset(a, 0, n * 8);
+  (void) hipEventCreate(&e0);
+  (void) hipEventCreate(&e1);
+  launch_copy_probe(a, b, n, nullptr);  // warm
+  (void) hipEventRecord(e0, nullptr);
+  for (int r = 0; r < reps; r++) launch_copy_probe(a, b, n, nullptr);
+  (void) hipEventRecord(e1, nullptr);
+  (void) hipEventSynchronize(e1);
+  float ms = 0;
+  (void) hipEventElapsedTime(&ms, e0, e1);
+  (void) hipEventDestroy(e0);
+  (void) hipEventDestroy(e1);
+  (void) hipFree(a);
+  (void) hipFree(b);
+  // bytes moved per launch: n*8 read + n*8 written
+  return 2.0 * n * 8 * reps / (ms * 1e-3) / 1e9;
+}
+
 // ---- reference call surface ------------------------------------------------------------
 
 // Globals the reference's driver expects (main.c:69, perfcnt.c); weak so a

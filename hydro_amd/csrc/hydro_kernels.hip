@@ -150,6 +150,21 @@ __global__ __launch_bounds__(256) void k_init(GridGeom g, GridView<S> a, GridVie
   init_cell<S>(g, a, b, t, bx, by0, by1, eb);
 }
 
+// ---- plain copy (measurement reference) ---------------------------------------
+
+// Streams n doubles from a to b; used to calibrate HBM byte counters and as
+// the achievable-copy-bandwidth reference.
+__global__ __launch_bounds__(256) void k_copy_probe(const double* __restrict__ a,
+                                                    double* __restrict__ b, long n) {
+  for (long k = (long) blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (long) gridDim.x * blockDim.x)
+    b[k] = a[k];
+}
+
+void launch_copy_probe(const double* a, double* b, long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, st, a, b, n);
+}
+
 // ---- launchers -----------------------------------------------------------------
 
 static inline unsigned cdiv(long a, long b) { return (unsigned) ((a + b - 1) / b); }

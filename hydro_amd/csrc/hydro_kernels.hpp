@@ -14,7 +14,7 @@ struct RunCtl {
   int active;    // 1 while the current step runs (t < tend && nstep < nstepmax)
   int log_base;  // dt of step n goes to log[n - log_base] when in range
   int log_cap;
-  int pad;
+  int fault;     // batches: set when a computed dt was not positive and finite
   double* log;
 };
 
@@ -84,5 +84,7 @@ void launch_clear_slots(uint64_t* cour, hipStream_t st);
 template <typename S>
 void launch_init(const GridGeom& g, GridView<S> a, GridView<S> b, int bx, int by0, int by1,
                  double e_blast, hipStream_t st);
+
+void launch_copy_probe(const double* a, double* b, long n, hipStream_t st);
 
 }  // namespace hydro

@@ -20,6 +20,7 @@
 #include "../../include/hydro_amd.h"
 
 int hydro_amd_set_error(const char* fmt, ...);  // hydro_capi.cpp
+int hydro_amd_batch_download_raw(hydro_amd_batch* b, int id, void* dst, int* elem_size);
 
 namespace {
 enum { QUEUED = 0, RUNNING = 1, DONE = 2, FAILED = 3 };
@@ -33,7 +34,8 @@ struct Request {
   int inst = -1;
   double t_sim = 0, dt = 0;
   int nstep = 0;
-  std::vector<double> result;
+  std::vector<unsigned char> result;  // state as stored (8- or 4-byte elements)
+  int esz = 0;
 };
 
 double now_s() {
@@ -98,8 +100,9 @@ extern "C" int hydro_amd_server_submit(hydro_amd_server* s, const hydro_amd_para
 static int retire(hydro_amd_server* s, int inst, int rid, int state) {
   Request& r = s->req[rid];
   if (s->keep && state == DONE) {
-    r.result.resize((size_t) 4 * (r.p.nx + 4) * (r.p.ny + 4));
-    if (hydro_amd_batch_download(s->b, inst, r.result.data())) return -1;
+    const size_t n = (size_t) 4 * (r.p.nx + 4) * (r.p.ny + 4);
+    r.result.resize(n * (r.p.precision == HYDRO_AMD_FP64 ? 8 : 4));
+    if (hydro_amd_batch_download_raw(s->b, inst, r.result.data(), &r.esz)) return -1;
   }
   if (hydro_amd_batch_remove(s->b, inst)) return -1;
   r.state = state;
@@ -152,8 +155,8 @@ extern "C" int hydro_amd_server_pump(hydro_amd_server* s) {
     r.t_sim = t[k];
     r.nstep = nstep[k];
     r.dt = dt[k];
-    const bool sane = nstep[k] == 0 || (dt[k] > 0 && isfinite(dt[k]) && isfinite(t[k]));
-    if (!sane || !more[k]) {
+    const bool sane = more[k] >= 0 && isfinite(t[k]);
+    if (!sane || more[k] == 0) {
       if (retire(s, ids[k], rid, sane ? DONE : FAILED)) return -1;
       s->running.erase(it);
     }
@@ -191,13 +194,20 @@ extern "C" int hydro_amd_server_result(hydro_amd_server* s, int rid, double* uol
   if (!s || rid < 0 || rid >= (int) s->req.size()) return hydro_amd_set_error("no request %d", rid);
   Request& r = s->req[rid];
   if (r.state != DONE || r.result.empty()) return hydro_amd_set_error("no result for %d", rid);
-  for (size_t k = 0; k < r.result.size(); k++) uold[k] = r.result[k];
+  const size_t n = r.result.size() / r.esz;
+  if (r.esz == 8) {
+    const double* d = (const double*) r.result.data();
+    for (size_t k = 0; k < n; k++) uold[k] = d[k];
+  } else {
+    const float* f = (const float*) r.result.data();
+    for (size_t k = 0; k < n; k++) uold[k] = f[k];
+  }
   return 0;
 }
 
 // Drops a kept result (frees host memory).
 extern "C" int hydro_amd_server_release(hydro_amd_server* s, int rid) {
   if (!s || rid < 0 || rid >= (int) s->req.size()) return hydro_amd_set_error("no request %d", rid);
-  std::vector<double>().swap(s->req[rid].result);
+  std::vector<unsigned char>().swap(s->req[rid].result);
   return 0;
 }

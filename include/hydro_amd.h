@@ -127,9 +127,12 @@ int hydro_amd_batch_download(hydro_amd_batch *b, int id, double *uold);
 int hydro_amd_batch_run(hydro_amd_batch *b, int steps);
 int hydro_amd_batch_run_async(hydro_amd_batch *b, int steps);
 int hydro_amd_batch_sync(hydro_amd_batch *b);
-/* Waits, then reports the live instances: ids, t, nstep, dt and whether
- * another step would open (t < tend && nstep < nstepmax).  Fills up to cap
- * entries; returns the number of live instances. */
+/* Waits, then reports the live instances: ids, t, nstep, dt and `active`:
+ * 1 if another step would open (t < tend && nstep < nstepmax), 0 if the run
+ * has ended, -1 if it was stopped at a dt that was not positive and finite
+ * (the batch stops such a run; the reference would go on, so this differs
+ * only for runs that have already broken down).  Fills up to cap entries;
+ * returns the number of live instances. */
 int hydro_amd_batch_query(hydro_amd_batch *b, int *ids, double *t, int *nstep, double *dt,
                           int *active, int cap);
 /* Task shape (waves per workgroup 1..4, segments per wave in x tasks, rows per
@@ -166,6 +169,12 @@ int hydro_amd_server_counts(hydro_amd_server *s, int *queued, int *running, int 
 int hydro_amd_server_info(hydro_amd_server *s, int req, double *times, double *t_sim, int *nstep);
 int hydro_amd_server_result(hydro_amd_server *s, int req, double *uold);
 int hydro_amd_server_release(hydro_amd_server *s, int req);
+
+/* Measurement support (no counterpart in the reference): runs a plain copy
+ * kernel (k_copy_probe) over `bytes` bytes `reps` times and returns GB/s
+ * counting bytes read + written; used as the achievable-bandwidth reference
+ * and to calibrate HBM byte counters. */
+double hydro_amd_copy_probe(long long bytes, int reps);
 
 /* Host mirror of the reference surface (hydro_surface.h): refresh Hv->uold
  * from the GPU, or push host edits of Hv->uold to the GPU. */

@@ -119,7 +119,7 @@ def test_instances_stop_at_their_own_limits():
             assert q[i][:3] == (t, n, dt)
             check_same(b.download(i), u_ref, f"instance {i}")
         assert [q[i][1] for i in ids] == [5, 12, 9, 21]
-        assert [q[i][3] for i in ids] == [False, False, False, True]
+        assert [q[i][3] for i in ids] == [0, 0, 0, 1]
 
 
 def test_budget_admission():
