@@ -185,6 +185,8 @@ extern "C" int hydro_amd_batch_add(hydro_amd_batch* b, const hydro_amd_params* p
   if (p->nx < 1 || p->ny < 1 || !(p->dx > 0) || !(p->gamma > 1) || p->scheme < 1 ||
       p->scheme > 3 || p->niter_riemann < 0)
     return hydro_amd_set_error("invalid instance parameters");
+  if ((long) ((p->nx + kXSeg - 1) / kXSeg) * p->ny > (1L << 30))
+    return hydro_amd_set_error("instance too large");
   if (p->precision != b->prec) return hydro_amd_set_error("instance precision differs from batch");
   int id = -1;
   for (int k = 0; k < kMaxInst; k++)
@@ -381,6 +383,15 @@ static void tasks_of(const hydro_amd_batch* b, int id, int kind, std::vector<BTa
   }
 }
 
+// Ghost fills of one instance in direction idim, kBoundaryChunk (line, var)
+// pairs per task so a large instance spreads over many workgroups.
+static void boundary_tasks_of(const hydro_amd_batch* b, int id, int idim,
+                              std::vector<BTask>& out) {
+  const Inst& in = b->inst[id];
+  const int nl = idim == 1 ? in.p.ny : in.p.nx;
+  for (int a = 0; a < 4 * nl; a += kBoundaryChunk) out.push_back(BTask{id, idim, a, 0});
+}
+
 // Reads the run controls of ids [0, maxid] into the pinned host copy.
 static int read_ctl(hydro_amd_batch* b) {
   if (b->maxid < 0) return 0;
@@ -407,8 +418,8 @@ static int rebuild(hydro_amd_batch* b) {
     const bool even = (b->hctl[id].nstep % 2) == 0;
     tasks_of(b, id, 1, even ? L[0] : L[2]);
     tasks_of(b, id, 2, even ? L[3] : L[1]);
-    L[4].push_back(BTask{id, even ? 1 : 2, 0, 0});
-    L[5].push_back(BTask{id, even ? 2 : 1, 0, 0});
+    boundary_tasks_of(b, id, even ? 1 : 2, L[4]);
+    boundary_tasks_of(b, id, even ? 2 : 1, L[5]);
   }
   int need = 0;
   for (int k = 0; k < 6; k++) {

@@ -23,7 +23,7 @@ template <typename S>
 __global__ __launch_bounds__(256) void k_batch_sweep_x(const InstDev<S>* inst, const BTask* tasks,
                                                        int ntask, int phase, int xiter) {
   const int wpb = blockDim.x >> 6;
-  const int wave = threadIdx.x >> 6;
+  const int wave = wave_id();
   for (int t = blockIdx.x; t < ntask; t += gridDim.x) {
     const BTask tk = tasks[t];
     const InstDev<S>& d = inst[tk.inst];
@@ -36,19 +36,17 @@ __global__ __launch_bounds__(256) void k_batch_sweep_x(const InstDev<S>* inst, c
     const bool cour = phase == 1;
     const bool st1 = d.ph.slope_type == (S) 1;
     double cx = 0.0, cy = 0.0;
-    const long ntot = (long) d.nseg * d.g.ny;
-    for (int it = 0; it < xiter; it++) {
-      const long s = (long) tk.a + (long) it * wpb + wave;
-      const bool ok = s < ntot;
-      const long jrow = ok ? s / d.nseg : 0;
-      const int seg = ok ? (int) (s - jrow * d.nseg) : 0;
+    // segments tk.a + it * wpb + wave, it = 0 .. xiter-1
+    SegWalk w(tk.a + wave, d.nseg);
+    for (int it = 0; it < xiter && w.row < d.g.ny; it++) {
       if (st1) {
-        if (cour) x_segment<S, true, true>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
-        else x_segment<S, true, false>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
+        if (cour) x_segment<S, true, true>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx, cy);
+        else x_segment<S, true, false>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx, cy);
       } else {
-        if (cour) x_segment<S, false, true>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
-        else x_segment<S, false, false>(d.g, d.ph, tr, dtdx, in, out, ok, jrow, seg, cx, cy);
+        if (cour) x_segment<S, false, true>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx, cy);
+        else x_segment<S, false, false>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx, cy);
       }
+      w.advance(wpb, d.nseg);
     }
     if (cour) commit_courant(cx, cy, d.cour, kInstSlots, t % kInstSlots);
   }
@@ -60,7 +58,7 @@ template <typename S>
 __global__ __launch_bounds__(256) void k_batch_sweep_y(const InstDev<S>* inst, const BTask* tasks,
                                                        int ntask, int phase, int yseg) {
   const int wpb = blockDim.x >> 6;
-  const int wave = threadIdx.x >> 6;
+  const int wave = wave_id();
   for (int t = blockIdx.x; t < ntask; t += gridDim.x) {
     const BTask tk = tasks[t];
     const InstDev<S>& d = inst[tk.inst];
@@ -83,8 +81,8 @@ __global__ __launch_bounds__(256) void k_batch_sweep_y(const InstDev<S>* inst, c
   }
 }
 
-// Ghost fill of one instance per task (kind = direction), from buffer `phase`
-// into both buffers.
+// Ghost fills (kind = direction) of (line, var) pairs [a, a + kBoundaryChunk)
+// of one instance per task, from buffer `phase` into both buffers.
 template <typename S>
 __global__ __launch_bounds__(256) void k_batch_boundary(const InstDev<S>* inst,
                                                         const BTask* tasks, int ntask,
@@ -96,7 +94,8 @@ __global__ __launch_bounds__(256) void k_batch_boundary(const InstDev<S>* inst,
     const int idim = tk.kind;
     const int nl = (idim == 1) ? d.g.ny : d.g.nx;
     const GridView<S> src = d.buf[phase], dst = d.buf[1 - phase];
-    for (int k = threadIdx.x; k < 4 * nl; k += blockDim.x) {
+    const int kend = min(tk.a + kBoundaryChunk, 4 * nl);
+    for (int k = tk.a + threadIdx.x; k < kend; k += blockDim.x) {
       const int line = k % nl, v = k / nl;
       boundary_line<S>(idim, d.g, d.bl, d.br, d.bd, d.bu, src.p[v], dst.p[v], v, line);
     }

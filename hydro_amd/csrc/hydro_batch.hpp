@@ -5,7 +5,8 @@
 
 namespace hydro {
 
-constexpr int kInstSlots = 64;  // CFL max slots per instance
+constexpr int kInstSlots = 64;        // CFL max slots per instance
+constexpr int kBoundaryChunk = 1024;  // (line, var) ghost fills per boundary task
 
 // One instance as the batched kernels see it (device array, rebuilt by the
 // host whenever the set of instances changes).

@@ -230,7 +230,10 @@ static int alloc_buffer(hydro_amd_grid* g, int k) {
 static int validate(const hydro_amd_params* p) {
   if (!p) return fail("null params");
   if (p->nx < 1 || p->ny < 1) return fail("nx, ny must be >= 1 (got %d, %d)", p->nx, p->ny);
-  if ((long) (p->nx + 4) * (p->ny + 4) > (1L << 40)) return fail("grid too large");
+  // x-sweep row segments are counted in 32-bit ints (hydro_sweeps.hpp SegWalk)
+  if ((long) (p->nx + 4) * (p->ny + 4) > (1L << 40) ||
+      (long) ((p->nx + kXSeg - 1) / kXSeg) * p->ny > (1L << 30))
+    return fail("grid too large");
   if (!(p->dx > 0)) return fail("dx must be > 0");
   if (!(p->gamma > 1)) return fail("gamma must be > 1");
   if (p->precision != HYDRO_AMD_FP64 && p->precision != HYDRO_AMD_MIXED)

@@ -23,14 +23,12 @@ __global__ __launch_bounds__(256) void k_sweep_x(GridGeom g, Phys<S> ph, double 
   const double dtdx = dt / dx;  // hydro_godunov.c:101
   const TraceK<S> tk = make_tracek<S>((S) dtdx, ph.scheme);
   const int wpb = blockDim.x >> 6;
-  const long ntask = (long) nseg * g.ny;
   double cx = 0.0, cy = 0.0;
-  for (int it = 0; it < xiter; it++) {
-    const long task = ((long) blockIdx.x * xiter + it) * wpb + (threadIdx.x >> 6);
-    const bool ok = task < ntask;
-    const long jrow = ok ? task / nseg : 0;
-    const int seg = ok ? (int) (task - jrow * nseg) : 0;
-    x_segment<S, ST1, COUR>(g, ph, tk, dtdx, in, out, ok, jrow, seg, cx, cy);
+  // segments (blockIdx.x * xiter + it) * wpb + wave, it = 0 .. xiter-1
+  SegWalk w((int) blockIdx.x * xiter * wpb + wave_id(), nseg);
+  for (int it = 0; it < xiter && w.row < g.ny; it++) {
+    x_segment<S, ST1, COUR>(g, ph, tk, dtdx, in, out, w.row, w.seg, cx, cy);
+    w.advance(wpb, nseg);
   }
   if (COUR) commit_courant(cx, cy, cour, kCourSlots, blockIdx.x % kCourSlots);
 }

@@ -64,14 +64,100 @@ struct Prim {
 template <typename R>
 __device__ __forceinline__ R maxm(R x, R y) { return x > y ? x : y; }  // utils.h:47 MAX
 
+// True when p holds on every active lane of the wave (a wave-uniform value).
+__device__ __forceinline__ bool wave_all(bool p) { return __builtin_amdgcn_ballot_w64(!p) == 0; }
+
+// Correctly rounded FP64 square root, fast form.  The compiler lowers sqrt(x)
+// to: x' = x * 2^256 if x < 2^-767; the v_rsq_f64 estimate g of x'; two
+// coupled Newton-Raphson refinements of x'*g and g/2 (sqrt_core below, op for
+// op); the result scaled back by 2^-128 if x was scaled; and x' itself
+// returned for +-0 and +inf.  For x in [2^-767, DBL_MAX] both scalings are by
+// 2^0 and the select is not taken, so sqrt(x) is sqrt_core(x) bit for bit.
+// sqrt_fast takes the core alone when every active lane of the wave is in
+// that range -- the normal case: the arguments here are gamma*p*rho-like
+// products floored by smallp and smallr -- and the full sequence otherwise.
+__device__ __forceinline__ double sqrt_core(double x) {
+  const double g = __builtin_amdgcn_rsq(x);
+  double s = x * g;
+  double h = g * 0.5;
+  const double r = __builtin_fma(-h, s, 0.5);
+  s = __builtin_fma(s, r, s);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-s, s, x);
+  s = __builtin_fma(d, h, s);
+  d = __builtin_fma(-s, s, x);
+  return __builtin_fma(d, h, s);
+}
+__device__ __forceinline__ bool sqrt_core_exact(double x) {
+  // high word in [hi(2^-767), hi(DBL_MAX)]: 2^-767 <= x <= DBL_MAX, x not NaN
+  const unsigned hi = (unsigned) __double2hiint(x);
+  return hi - 0x10000000u <= 0x7fefffffu - 0x10000000u;
+}
+__device__ __forceinline__ double sqrt_fast(double x) {
+  if (wave_all(sqrt_core_exact(x))) return sqrt_core(x);
+  return sqrt(x);
+}
+__device__ __forceinline__ void sqrt_fast2(double x, double y, double& sx, double& sy) {
+  if (wave_all(sqrt_core_exact(x) && sqrt_core_exact(y))) {
+    sx = sqrt_core(x);
+    sy = sqrt_core(y);
+  } else {
+    sx = sqrt(x);
+    sy = sqrt(y);
+  }
+}
+
 // Division and square root.  FP64: the IEEE operations themselves (what the
 // reference computes).  FP32 (mixed precision only): the hardware reciprocal
 // and square root (v_rcp_f32 / v_sqrt_f32, about 1 ulp each) instead of the
 // correctly rounded multi-instruction sequences.
 __device__ __forceinline__ double xdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float xdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-__device__ __forceinline__ double xsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ double xsqrt(double a) { return sqrt_fast(a); }
 __device__ __forceinline__ float xsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+__device__ __forceinline__ void xsqrt2(double a, double b, double& sa, double& sb) {
+  sqrt_fast2(a, b, sa, sb);
+}
+__device__ __forceinline__ void xsqrt2(float a, float b, float& sa, float& sb) {
+  sa = __builtin_amdgcn_sqrtf(a);
+  sb = __builtin_amdgcn_sqrtf(b);
+}
+
+// FP64 division and square roots of the Riemann first guess and Newton
+// iteration, by the sweep's compute type R.  R = double (the FP64 form): the
+// correctly rounded operations.  R = float (mixed precision): the hardware
+// estimates v_rcp_f64 / v_rsq_f64 refined by Newton-Raphson steps
+// (relative error a few ulp of FP64, not correctly rounded).
+__device__ __forceinline__ double div_nr(double a, double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  y = __builtin_fma(__builtin_fma(-b, y, 1.0), y, y);
+  const double q = a * y;
+  return __builtin_fma(__builtin_fma(-b, q, a), y, q);
+}
+__device__ __forceinline__ double sqrt_nr(double x) {
+  const double g = __builtin_amdgcn_rsq(x);
+  const double s = x * g;
+  const double r = __builtin_fma(-(g * 0.5), s, 0.5);
+  const double v = __builtin_fma(s, r, s);
+  return x == 0.0 ? x : v;  // rsq(0) = inf
+}
+template <typename R>
+__device__ __forceinline__ double ndiv(double a, double b);
+template <>
+__device__ __forceinline__ double ndiv<double>(double a, double b) { return a / b; }
+template <>
+__device__ __forceinline__ double ndiv<float>(double a, double b) { return div_nr(a, b); }
+template <typename R>
+__device__ __forceinline__ void nsqrt2(double x, double y, double& sx, double& sy);
+template <>
+__device__ __forceinline__ void nsqrt2<double>(double x, double y, double& sx, double& sy) {
+  sqrt_fast2(x, y, sx, sy);
+}
+template <>
+__device__ __forceinline__ void nsqrt2<float>(double x, double y, double& sx, double& sy) {
+  sx = sqrt_nr(x);
+  sy = sqrt_nr(y);
+}
 
 // constoprim.c:62-78 + equation_of_state.c:64-81 for one cell.  U is in sweep
 // slot order (slot 1 = normal momentum).
@@ -225,30 +311,32 @@ __device__ __forceinline__ Q4<R> riemann_flux(const Q4<R>& ql, const Q4<R>& qr,
 
   // First guess and Newton iteration for p* (riemann.c:145-201), in double.
   const double dpl = pl, dpr = pr, dcl = cl, dcr = cr, dul = ul, dur = ur;
-  double wl0 = sqrt(dcl);
-  double wr0 = sqrt(dcr);
-  double pst = fmax(((wr0 * dpl + wl0 * dpr) + wl0 * wr0 * (dul - dur)) / (wl0 + wr0), 0.0);
+  double wl0, wr0;
+  nsqrt2<R>(dcl, dcr, wl0, wr0);
+  double pst = fmax(ndiv<R>((wr0 * dpl + wl0 * dpr) + wl0 * wr0 * (dul - dur), wl0 + wr0), 0.0);
   const double g6 = ph.gamma6_d;
   for (int it = 0; it < ph.niter; it++) {
-    double wwl = sqrt(dcl * (1.0 + g6 * (pst - dpl) / dpl));
-    double wwr = sqrt(dcr * (1.0 + g6 * (pst - dpr) / dpr));
+    double wwl, wwr;
+    nsqrt2<R>(dcl * (1.0 + ndiv<R>(g6 * (pst - dpl), dpl)),
+              dcr * (1.0 + ndiv<R>(g6 * (pst - dpr), dpr)), wwl, wwr);
     double swwl = wwl * wwl;
-    double qlw = 2.0 * wwl * swwl / (swwl + dcl);
-    double qrw = 2.0 * wwr * (wwr * wwr) / ((wwr * wwr) + dcr);
-    double usl = dul - (pst - dpl) / wwl;
-    double usr = dur + (pst - dpr) / wwr;
-    double t = (qrw * qlw / (qrw + qlw) * (usl - usr));
+    double qlw = ndiv<R>(2.0 * wwl * swwl, swwl + dcl);
+    double qrw = ndiv<R>(2.0 * wwr * (wwr * wwr), (wwr * wwr) + dcr);
+    double usl = dul - ndiv<R>(pst - dpl, wwl);
+    double usr = dur + ndiv<R>(pst - dpr, wwr);
+    double t = (ndiv<R>(qrw * qlw, qrw + qlw) * (usl - usr));
     double d = (t > (-pst)) ? t : (-pst);
     pst += d;
-    double rel = d / (pst + ph.smallpp_d);
+    double rel = ndiv<R>(d, pst + ph.smallpp_d);
     double arel = (rel > 0) ? rel : -rel;
     if (!(arel > 1e-6)) break;
   }
 
   // Star state and sampling at x/t = 0 (riemann.c:211-276).
   R ps = (R) pst;
-  R wr = xsqrt(cr * (one + xdiv((R) g6 * (ps - pr), pr)));
-  R wl = xsqrt(cl * (one + xdiv((R) g6 * (ps - pl), pl)));
+  R wr, wl;
+  xsqrt2(cr * (one + xdiv((R) g6 * (ps - pr), pr)), cl * (one + xdiv((R) g6 * (ps - pl), pl)),
+         wr, wl);
   R ustar = half * (ul + xdiv(pl - ps, wl) + ur - xdiv(pr - ps, wr));
   bool left = ustar > (R) 0;
   R sg = left ? (R) 1 : (R) -1;
@@ -256,11 +344,11 @@ __device__ __forceinline__ Q4<R> riemann_flux(const Q4<R>& ql, const Q4<R>& qr,
   R uo = left ? ul : ur;
   R po = left ? pl : pr;
   R wo = left ? wl : wr;
-  R co = xsqrt(fabs(xdiv(ph.gamma * po, ro)));
-  co = fmax(ph.smallc, co);
   R rstar = xdiv(ro, one + xdiv(ro * (po - ps), wo * wo));
   rstar = fmax(rstar, ph.smallr);
-  R cstar = xsqrt(fabs(xdiv(ph.gamma * ps, rstar)));
+  R co, cstar;  // (riemann.c computes co before rstar; neither depends on the other)
+  xsqrt2(fabs(xdiv(ph.gamma * po, ro)), fabs(xdiv(ph.gamma * ps, rstar)), co, cstar);
+  co = fmax(ph.smallc, co);
   cstar = fmax(ph.smallc, cstar);
   R spout = co - sg * uo;
   R spin = cstar - sg * ustar;

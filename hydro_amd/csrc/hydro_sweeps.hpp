@@ -111,11 +111,34 @@ __device__ __forceinline__ double wave_reduce_dt(uint64_t* cour, int nslot, doub
 
 // ---- x sweep: one 60-cell segment of one row per wave ----------------------
 
+// The wave's index in its workgroup, as a wave-uniform (scalar) value.
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane((int) (threadIdx.x >> 6));
+}
+
+// Row segments s0, s0 + stride, s0 + 2*stride, ... of a grid with nseg
+// segments per row, as (row, segment) pairs: one division up front, then
+// carries.  Every value is wave-uniform; callers keep nseg * rows < 2^31.
+struct SegWalk {
+  int row, seg;
+  __device__ __forceinline__ SegWalk(int s0, int nseg) {
+    row = s0 / nseg;
+    seg = s0 - row * nseg;
+  }
+  __device__ __forceinline__ void advance(int stride, int nseg) {
+    seg += stride;
+    while (seg >= nseg) {
+      seg -= nseg;
+      row++;
+    }
+  }
+};
+
 template <typename S, bool ST1, bool COUR>
 __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
                                           const TraceK<S>& tk, double dtdx,
                                           const GridView<S>& in, const GridView<S>& out,
-                                          bool ok, long jrow, int seg, double& cx, double& cy) {
+                                          long jrow, int seg, double& cx, double& cy) {
   const int lane = threadIdx.x & 63;
   const long j = jrow + 2;
   const int i = kXSeg * seg + lane;  // padded column of this lane
@@ -136,7 +159,7 @@ __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
   const Q4<S> fr = riemann_flux<S>(qxm, qr, ph);
   const Q4<S> fl = from_left4(fr);
 
-  const bool wr = ok && lane >= 2 && lane <= kXSeg + 1 && i <= g.nx + 1;
+  const bool wr = lane >= 2 && lane <= kXSeg + 1 && i <= g.nx + 1;
   const double n0 = update1<S>(U0, fl.r, fr.r, dtdx);
   const double n1 = update1<S>(U1, fl.u, fr.u, dtdx);
   const double n2 = update1<S>(U2, fl.v, fr.v, dtdx);
@@ -199,7 +222,8 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
   S cD = pd.c;
   Q4<S> qxmC = qxm_c;
   Cell<S> uC = uc, uD = ud;
-  for (int j = j0 + 2; j <= j1 + 1; j++) {
+  // One row of the march: load row j, update row j-2, shift the window.
+  auto row = [&](int j) {
     const long oj = (long) j * P + i;
     const Cell<S> uE = load_y(in, oj);
     const Prim<S> pE = prim_cell<S>(uE.u0, uE.u1, uE.u2, uE.u3, ph);
@@ -231,7 +255,15 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
     cD = pE.c;
     uC = uD;
     uD = uE;
+  };
+  // two rows per trip: the window shifts by two, so its registers map back
+  // onto themselves without copies
+  int j = j0 + 2;
+  for (; j + 1 <= j1 + 1; j += 2) {
+    row(j);
+    row(j + 1);
   }
+  if (j <= j1 + 1) row(j);
 }
 
 // ---- ghost cells -------------------------------------------------------------
