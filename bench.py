@@ -114,39 +114,59 @@ def batched_bench(n=512, steps=100):
     return out
 
 
-def serving_bench(n=32, rate=20.0, chunk=20):
-    """Config 4: n runs (sides log-uniform in [128, 4096], t_end 0.1 or 500
-    steps) arriving as a Poisson process; mixed precision served by the
-    continuous-batching loop against the plain per-grid FP64 loop (FIFO, one
-    run at a time) on the same requests.  Latency = finished - arrived."""
-    import numpy as np
+def _serve(runs, chunk):
+    """Feed `runs` to a mixed-precision Server at their arrival offsets (real
+    time) and pump it until every run is done; returns (server, request ids,
+    per-request info, span in seconds from the first arrival)."""
     import hydro_amd
-    from hydro_amd import workloads as W
-    runs = W.serving_requests(n, rate=rate, precision=hydro_amd.MIXED)
     srv = hydro_amd.Server(precision=hydro_amd.MIXED, budget_bytes=200 * 10**9,
                            chunk_steps=chunk, keep_results=True)
     t_start = hydro_amd.now()
-    ids = []
-    k = 0
-    while k < n or srv.pump() > 0:
+    ids, k, n = [], 0, len(runs)
+    while True:
         now = hydro_amd.now() - t_start
         while k < n and runs[k].arrival <= now:
             ids.append(srv.submit(runs[k].params, runs[k].bx, runs[k].by, runs[k].e))
             k += 1
-        if k < n and srv.counts()["running"] == 0 and srv.counts()["queued"] == 0:
-            time.sleep(max(0.0, min(0.01, runs[k].arrival - now)))
+        left = srv.pump()  # admit, advance everything running by one chunk, retire
+        if left == 0:
+            if k == n:
+                break
+            time.sleep(max(0.0, runs[k].arrival - (hydro_amd.now() - t_start)))
     info = [srv.info(i) for i in ids]
     assert all(x[0] == srv.DONE for x in info), [x[0] for x in info]
-    lat = np.array([x[3] - x[1] for x in info])
-    span = max(x[3] for x in info) - t_start
-    res = {"requests": n, "rate_per_s": rate, "chunk_steps": chunk,
-           "served_mixed": {"solves_per_s": n / span, "seconds": span,
-                            "p50_s": float(np.percentile(lat, 50)),
-                            "p99_s": float(np.percentile(lat, 99)),
-                            "steps": [x[5] for x in info][:8]}}
-    # the plain per-grid FP64 loop on the same requests, FIFO
+    return srv, ids, info, max(x[3] for x in info) - t_start
+
+
+def serving_bench(n=32, rates=(20.0, 1000.0), chunk=20):
+    """Config 4: n runs (sides log-uniform in [128, 4096], t_end 0.1 or 500
+    steps) arriving as a Poisson process; mixed precision served by the
+    continuous-batching loop against the plain per-grid FP64 loop (FIFO, one
+    run at a time, each run's service time measured) on the same requests.
+    Latency = finished - arrived (results on the host in both).  The first
+    rate is the nominal one; the last is far above either's capacity, so its
+    solves/s is each side's throughput under a full queue."""
+    import numpy as np
+    import hydro_amd
+    from hydro_amd import workloads as W
+    res = {"requests": n, "chunk_steps": chunk, "rates": []}
+    served = []
+    for i, rate in enumerate(rates):
+        runs = W.serving_requests(n, rate=rate, precision=hydro_amd.MIXED)
+        srv, ids, info, span = _serve(runs, chunk)
+        lat = np.array([x[3] - x[1] for x in info])
+        served.append({"rate_per_s": rate, "runs": runs,
+                       "served_mixed": {"solves_per_s": n / span, "seconds": span,
+                                        "p50_s": float(np.percentile(lat, 50)),
+                                        "p99_s": float(np.percentile(lat, 99))}})
+        if i == 0:
+            keep_srv, keep_ids = srv, ids
+        else:
+            srv.close()
+    # the plain per-grid FP64 loop: service time of each run alone (create,
+    # run to t_end / nstepmax, download), and the mixed results' error
     service, errs = [], []
-    for rid, r in zip(ids, runs):
+    for rid, r in zip(keep_ids, served[0]["runs"]):
         p64 = hydro_amd.Params(**{**r.params.__dict__, "precision": hydro_amd.FP64})
         t0 = time.perf_counter()
         with hydro_amd.Grid(p64) as g:
@@ -154,8 +174,8 @@ def serving_bench(n=32, rate=20.0, chunk=20):
             g.run(10**6, log=False)
             u64 = g.download()
         service.append(time.perf_counter() - t0)
-        um = srv.result(rid)
-        srv.release(rid)
+        um = keep_srv.result(rid)
+        keep_srv.release(rid)
         e = []
         for v in (0, 3):
             a, b = um[v, 2:-2, 2:-2], u64[v, 2:-2, 2:-2]
@@ -163,16 +183,19 @@ def serving_bench(n=32, rate=20.0, chunk=20):
                       float(np.abs(a - b).max() / np.abs(b).max())))
         errs.append(e)
         del um, u64
-    srv.close()
-    done, lat_plain = 0.0, []
-    for r, sv in zip(runs, service):  # FIFO queue of one server
-        done = max(done, r.arrival) + sv
-        lat_plain.append(done - r.arrival)
+    keep_srv.close()
+    for s in served:
+        done, lat_plain = 0.0, []
+        for r, sv in zip(s.pop("runs"), service):  # FIFO queue of one server
+            done = max(done, r.arrival) + sv
+            lat_plain.append(done - r.arrival)
+        s["plain_fp64_per_grid"] = {"solves_per_s": n / done, "seconds": done,
+                                    "p50_s": float(np.percentile(lat_plain, 50)),
+                                    "p99_s": float(np.percentile(lat_plain, 99))}
+        s["speedup_solves_per_s"] = s["served_mixed"]["solves_per_s"] / (n / done)
+        res["rates"].append(s)
     errs = np.array(errs)
-    res["plain_fp64_per_grid"] = {"solves_per_s": n / done, "seconds": done,
-                                  "p50_s": float(np.percentile(lat_plain, 50)),
-                                  "p99_s": float(np.percentile(lat_plain, 99))}
-    res["speedup_solves_per_s"] = res["served_mixed"]["solves_per_s"] / (n / done)
+    res["plain_fp64_service_s_total"] = float(sum(service))
     res["mixed_error_vs_fp64"] = {"density_l1_max": float(errs[:, 0, 0].max()),
                                   "density_linf_max": float(errs[:, 0, 1].max()),
                                   "energy_l1_max": float(errs[:, 1, 0].max()),
@@ -186,20 +209,23 @@ def cpu_threads():
     return max(1, min(n, int(os.environ.get("HYDRO_CPU_THREADS", "16"))))
 
 
-def cpu_baseline(nx, ny, nsteps=8):
+def cpu_baseline(nx, ny, nsteps=8, testcase=0):
     """Time the reference OpenMP CPU path on `nsteps` steps of the same grid.
 
     MC/s per the reference (HydroC99_2Domp/Src/main.c:426-427, averaged over
-    steps > 5); reported in cell-updates/s."""
+    steps > 5; with fewer steps, over every step after the first, which pays
+    the first touch of the arrays); reported in cell-updates/s."""
     ref = os.path.join(ROOT, "oracle", "_ref", "hydro_omp")
     threads = cpu_threads()
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread",
                OMP_PLACES="cores")
     deck = ("&RUN\ntend=1e30\nnstepmax=%d\n/\n&MESH\nnx=%d\nny=%d\nnxystep=1024\ndx=%r\n"
-            "boundary_left=1\nboundary_right=1\nboundary_down=1\nboundary_up=1\ntestcase=0\n/\n"
+            "boundary_left=1\nboundary_right=1\nboundary_down=1\nboundary_up=1\ntestcase=%d\n/\n"
             "&HYDRO\ncourant_factor=0.8\nniter_riemann=10\niorder=2\nslope_type=1.\n"
-            "scheme=muscl\nsmallr=1e-10\nsmallc=1e-10\n/\n" % (nsteps, nx, ny, 1.0 / nx))
-    sample = f"{nx}x{ny} centred blast, {nsteps} steps, MC/s averaged over steps > 5"
+            "scheme=muscl\nsmallr=1e-10\nsmallc=1e-10\n/\n" % (nsteps, nx, ny, 1.0 / nx, testcase))
+    blast = {0: "centred blast", 1: "corner blast"}.get(testcase, f"testcase {testcase}")
+    over = "steps > 5" if nsteps > 5 else "steps 2.." + str(nsteps)
+    sample = f"{nx}x{ny} {blast}, {nsteps} steps, MC/s averaged over {over}"
     if os.path.exists(ref):
         with tempfile.TemporaryDirectory() as d:
             path = os.path.join(d, "bench.nml")
@@ -209,33 +235,92 @@ def cpu_baseline(nx, ny, nsteps=8):
                                  text=True, timeout=600).stdout
         per_step = [float(m) for m in re.findall(r"--> step=\s*\d+,.*?([0-9.]+) MC/s", out)]
         avg = re.search(r"Average MC/s: ([0-9.]+)", out)
-        mcs = float(avg.group(1)) if avg else (sum(per_step[5:]) / max(1, len(per_step[5:])))
+        mcs = float(avg.group(1)) if avg else 0.0
+        if not mcs > 0:
+            tail = per_step[1:] or per_step
+            mcs = sum(tail) / max(1, len(tail))
         return {"value": mcs * 1e6, "unit": "cell-updates/s", "cores": threads,
                 "kind": "reference", "sample": sample + " (HydroC99_2Domp, gcc -O3 -fopenmp)"}
     import oracle
-    p = oracle.Params(nx=nx, ny=ny, dx=1.0 / nx, courant_factor=0.8, tend=1e30, testcase=0)
+    p = oracle.Params(nx=nx, ny=ny, dx=1.0 / nx, courant_factor=0.8, tend=1e30,
+                      testcase=testcase)
     port = oracle.Port()
     u = port.init(p)
     t0 = time.perf_counter()
     port.run(p, u, 2)
     dt = time.perf_counter() - t0
     return {"value": nx * ny * 2 / dt, "unit": "cell-updates/s", "cores": threads,
-            "kind": "port", "sample": f"{nx}x{ny} centred blast, 2 steps (oracle restatement)"}
+            "kind": "port", "sample": f"{nx}x{ny} {blast}, 2 steps (oracle restatement)"}
 
 
-def pmc_workload(nx, precision, steps):
+def pmc_workload(nx, precision, steps, tune=None):
     """A short fixed workload for counter runs (rocprofv3 --pmc): one plain
     copy kernel of 1 GiB (the byte-counter calibration), then `steps` steps of
-    the bench grid without the autotuner."""
+    the bench grid with the given launch shape (default shape if None)."""
     import hydro_amd
     gbs = hydro_amd.copy_probe(1 << 30, 1)
     p = hydro_amd.Params(nx=nx, ny=nx, dx=1.0 / nx, courant_factor=0.8, tend=1e30,
                          precision=precision)
     with hydro_amd.Grid(p) as g:
         g.init(0)
+        if tune:
+            g.set_tune(*tune)
         g.run(steps, log=False)
     print(json.dumps({"pmc_workload": True, "copy_bytes": 1 << 30, "copy_GBps": gbs,
                       "nx": nx, "steps": steps}), flush=True)
+
+
+COPY_PROBE_BYTES = 1 << 30  # each of read and written by pmc_workload's copy kernel
+
+
+def _pmc_kib(counter, args):
+    """Mean of one rocprofv3 counter (KiB per dispatch) per kernel over a run
+    of this script with `args`, or None if the profiler is absent or fails."""
+    import csv
+    import shutil
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+               sys.executable, os.path.abspath(__file__), "--pmc-workload"] + args
+        try:
+            subprocess.run(cmd, cwd=d, env=dict(os.environ, TMPDIR="/tmp"), timeout=150,
+                           capture_output=True, check=True)
+        except (subprocess.SubprocessError, OSError):
+            return None
+        path = os.path.join(d, "run_counter_collection.csv")
+        if not os.path.exists(path):
+            return None
+        acc = {}
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r.get("Counter_Name") != counter:
+                    continue
+                name = r["Kernel_Name"]
+                key = ("copy" if "k_copy_probe" in name else "x_sweep" if "k_sweep_x" in name
+                       else "y_sweep" if "k_sweep_y" in name else None)
+                if key:
+                    acc.setdefault(key, []).append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def measured_traffic(nx, precision, tune):
+    """HBM bytes per launch of each sweep kernel: rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes over pmc_workload, each scaled by what the same counter
+    reads for the copy kernel (COPY_PROBE_BYTES read and written)."""
+    args = ["--steps", "2", "--nx", str(nx), "--precision", ["fp64", "mixed"][precision]]
+    if tune:
+        args += ["--tune", ",".join(str(tune[k]) for k in ("yseg", "ywaves", "xwaves", "xiter"))]
+    fetch, write = _pmc_kib("FETCH_SIZE", args), _pmc_kib("WRITE_SIZE", args)
+    if not fetch or not write or "copy" not in fetch or "copy" not in write:
+        return None
+    out = {"fetch_scale": COPY_PROBE_BYTES / (fetch["copy"] * 1024),
+           "write_scale": COPY_PROBE_BYTES / (write["copy"] * 1024)}
+    for k in ("x_sweep", "y_sweep"):
+        if k in fetch and k in write:
+            out[k] = (fetch[k] * 1024 * out["fetch_scale"] + write[k] * 1024 * out["write_scale"])
+    return out
 
 
 def main():
@@ -257,6 +342,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--pmc-workload", action="store_true",
                     help="short fixed run for rocprofv3 counter passes")
+    ap.add_argument("--tune", default="", help="yseg,ywaves,xwaves,xiter (pmc workload)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes behind roofline.traffic")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); the "
@@ -265,7 +353,8 @@ def main():
     ny = a.ny or a.nx
     prec = 0 if a.precision == "fp64" else 1
     if a.pmc_workload:
-        pmc_workload(a.nx, prec, a.steps)
+        pmc_workload(a.nx, prec, a.steps,
+                     [int(v) for v in a.tune.split(",")] if a.tune else None)
         return
     r = gpu_bench(a.nx, ny, a.steps, a.warmup, prec, autotune=not a.no_autotune,
                   testcase=a.testcase)
@@ -291,12 +380,20 @@ def main():
         "roofline": r["roofline"],
         "kernels": r["kernels"],
     }
+    if not a.no_traffic:
+        tr = measured_traffic(a.nx, prec, r["tune"])
+        kern = line["roofline"]["kernel"]
+        if tr and kern in tr:
+            line["roofline"]["traffic"] = round(tr[kern])
+            line["roofline"]["traffic_note"] = (
+                "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by "
+                f"{tr['fetch_scale']:.3f} / {tr['write_scale']:.3f} (copy-kernel calibration)")
     if not a.no_batched:
         line["batched"] = batched_bench(a.batch_n)
     if not a.no_serving:
-        line["serving"] = serving_bench(a.serve_n, a.serve_rate)
+        line["serving"] = serving_bench(a.serve_n, (a.serve_rate, 1000.0))
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(a.nx, ny, a.cpu_steps)
+        line["cpu_baseline"] = cpu_baseline(a.nx, ny, a.cpu_steps, a.testcase)
     print(json.dumps(line), flush=True)
 
 

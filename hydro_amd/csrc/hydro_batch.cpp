@@ -288,18 +288,7 @@ extern "C" int hydro_amd_batch_download(hydro_amd_batch* b, int id, double* u) {
   if (!in || !u) return in ? hydro_amd_set_error("null argument") : -1;
   const long nxt = in->p.nx + 4, rows = 4L * (in->p.ny + 4);
   const void* src = (const char*) in->mem[0] + kOrigin * b->esz;
-  if (b->prec == HYDRO_AMD_FP64) {
-    BCHK(hipMemcpy2DAsync(u, nxt * 8, src, in->g.pitch * 8, nxt * 8, rows,
-                          hipMemcpyDeviceToHost, b->st));
-    BCHK(hipStreamSynchronize(b->st));
-  } else {
-    std::vector<float> tmp((size_t) nxt * rows);
-    BCHK(hipMemcpy2DAsync(tmp.data(), nxt * 4, src, in->g.pitch * 4, nxt * 4, rows,
-                          hipMemcpyDeviceToHost, b->st));
-    BCHK(hipStreamSynchronize(b->st));
-    for (size_t k = 0; k < tmp.size(); k++) u[k] = tmp[k];
-  }
-  return 0;
+  return stage_d2h(u, 8, src, (int) b->esz, in->g.pitch, nxt, rows, b->st) ? -1 : 0;
 }
 
 // Copies an instance's state as stored (element size 8 or 4), layout
@@ -310,9 +299,7 @@ int hydro_amd_batch_download_raw(hydro_amd_batch* b, int id, void* dst, int* ele
   const long nxt = in->p.nx + 4, rows = 4L * (in->p.ny + 4);
   const size_t w = b->esz;
   const void* src = (const char*) in->mem[0] + kOrigin * w;
-  BCHK(hipMemcpy2DAsync(dst, nxt * w, src, in->g.pitch * w, nxt * w, rows,
-                        hipMemcpyDeviceToHost, b->st));
-  BCHK(hipStreamSynchronize(b->st));
+  if (stage_d2h(dst, (int) w, src, (int) w, in->g.pitch, nxt, rows, b->st)) return -1;
   if (elem_size) *elem_size = (int) w;
   return 0;
 }

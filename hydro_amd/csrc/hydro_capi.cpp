@@ -70,6 +70,7 @@ struct hydro_amd_grid {
   GridView<float> vf[2];
   uint64_t* cour = nullptr;
   RunCtl* ctl = nullptr;
+  RunCtl* hctl = nullptr;   // pinned host copy: every RunCtl / dt transfer goes through it
   double* dscal = nullptr;  // device scalar for dt
   double* dlog = nullptr;
   int dlog_cap = 0;
@@ -209,6 +210,7 @@ extern "C" void hydro_amd_grid_destroy(hydro_amd_grid* g) {
   if (g->ctl) (void) hipFree(g->ctl);
   if (g->dscal) (void) hipFree(g->dscal);
   if (g->dlog) (void) hipFree(g->dlog);
+  if (g->hctl) (void) hipHostFree(g->hctl);
   if (g->st) (void) hipStreamDestroy(g->st);
   delete g;
 }
@@ -261,16 +263,18 @@ extern "C" hydro_amd_grid* hydro_amd_grid_create(const hydro_amd_params* p) {
       alloc_buffer(g, 0) || alloc_buffer(g, 1) ||
       hipMalloc(&g->cour, 2 * kCourSlots * sizeof(uint64_t)) != hipSuccess ||
       hipMalloc(&g->ctl, sizeof(RunCtl)) != hipSuccess ||
+      hipHostMalloc(&g->hctl, sizeof(RunCtl), hipHostMallocDefault) != hipSuccess ||
       hipMalloc(&g->dscal, sizeof(double)) != hipSuccess) {
     if (g_err.empty()) fail("device allocation failed");
     hydro_amd_grid_destroy(g);
     return nullptr;
   }
   launch_clear_slots(g->cour, g->st);
-  RunCtl c = {};
-  c.tend = p->tend;
-  c.nstepmax = p->nstepmax;
-  if (hipMemcpyAsync(g->ctl, &c, sizeof(c), hipMemcpyHostToDevice, g->st) != hipSuccess ||
+  *g->hctl = RunCtl{};
+  g->hctl->tend = p->tend;
+  g->hctl->nstepmax = p->nstepmax;
+  if (hipMemcpyAsync(g->ctl, g->hctl, sizeof(RunCtl), hipMemcpyHostToDevice, g->st) !=
+          hipSuccess ||
       hipStreamSynchronize(g->st) != hipSuccess) {
     fail("grid setup failed");
     hydro_amd_grid_destroy(g);
@@ -287,7 +291,9 @@ extern "C" long long hydro_amd_grid_bytes(const hydro_amd_grid* g) {
 }
 
 static int write_ctl(hydro_amd_grid* g) {
-  RunCtl c = {};
+  HIPCHK(hipStreamSynchronize(g->st));  // the pinned copy may still feed a queued copy
+  RunCtl& c = *g->hctl;
+  c = RunCtl{};
   c.t = g->t;
   c.dt = g->dt;
   c.tend = g->p.tend;
@@ -295,7 +301,7 @@ static int write_ctl(hydro_amd_grid* g) {
   c.nstepmax = g->p.nstepmax;
   c.active = 0;
   c.log = nullptr;
-  HIPCHK(hipMemcpyAsync(g->ctl, &c, sizeof(c), hipMemcpyHostToDevice, g->st));
+  HIPCHK(hipMemcpyAsync(g->ctl, g->hctl, sizeof(RunCtl), hipMemcpyHostToDevice, g->st));
   HIPCHK(hipStreamSynchronize(g->st));
   return 0;
 }
@@ -361,18 +367,7 @@ extern "C" int hydro_amd_grid_download(hydro_amd_grid* g, double* u) {
   if (!g || !u) return fail("null argument");
   const long nxt = g->p.nx + 4, rows = 4L * (g->p.ny + 4);
   const void* src = (const char*) g->mem[0] + kOrigin * g->esz;
-  if (g->prec == HYDRO_AMD_FP64) {
-    HIPCHK(hipMemcpy2DAsync(u, nxt * 8, src, g->geom.pitch * 8, nxt * 8, rows,
-                            hipMemcpyDeviceToHost, g->st));
-    HIPCHK(hipStreamSynchronize(g->st));
-  } else {
-    std::vector<float> tmp((size_t) nxt * rows);
-    HIPCHK(hipMemcpy2DAsync(tmp.data(), nxt * 4, src, g->geom.pitch * 4, nxt * 4, rows,
-                            hipMemcpyDeviceToHost, g->st));
-    HIPCHK(hipStreamSynchronize(g->st));
-    for (size_t k = 0; k < tmp.size(); k++) u[k] = tmp[k];
-  }
-  return 0;
+  return stage_d2h(u, 8, src, (int) g->esz, g->geom.pitch, nxt, rows, g->st) ? -1 : 0;
 }
 
 // ---- steps ---------------------------------------------------------------------------
@@ -470,8 +465,9 @@ extern "C" int hydro_amd_grid_compute_deltat(hydro_amd_grid* g, double* dt) {
   HIPCHK(hipGetLastError());
   g->cour_dirty = false;
   g->cour_ver = 0;
-  HIPCHK(hipMemcpyAsync(dt, g->dscal, sizeof(double), hipMemcpyDeviceToHost, g->st));
+  HIPCHK(hipMemcpyAsync(&g->hctl->dt, g->dscal, sizeof(double), hipMemcpyDeviceToHost, g->st));
   HIPCHK(hipStreamSynchronize(g->st));
+  *dt = g->hctl->dt;
   return 0;
 }
 
@@ -487,8 +483,9 @@ static void enq_step(hydro_amd_grid* g, int n) {
 }
 
 static int read_ctl(hydro_amd_grid* g, RunCtl* c) {
-  HIPCHK(hipMemcpyAsync(c, g->ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, g->st));
+  HIPCHK(hipMemcpyAsync(g->hctl, g->ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, g->st));
   HIPCHK(hipStreamSynchronize(g->st));
+  *c = *g->hctl;
   return 0;
 }
 
@@ -522,7 +519,10 @@ static int run_enqueue(hydro_amd_grid* g, int max_steps, int log_cap) {
   c.log_base = g->nstep;
   c.log_cap = log_cap;
   c.log = log_cap > 0 ? g->dlog : nullptr;
-  HIPCHK(hipMemcpyAsync(g->ctl, &c, sizeof(c), hipMemcpyHostToDevice, g->st));
+  // read_ctl left the stream idle, so the pinned copy is free; nothing
+  // rewrites it before the next synchronising call
+  *g->hctl = c;
+  HIPCHK(hipMemcpyAsync(g->ctl, g->hctl, sizeof(RunCtl), hipMemcpyHostToDevice, g->st));
   int n = g->nstep, left = max_steps;
   if (n % 2 == 1) {  // finish the odd step first so the graph starts even
     enq_step(g, n);

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void k_sweep_y(GridGeom g, Phys<S> ph, double 
   const double dtdx = dt / dx;
   const TraceK<S> tk = make_tracek<S>((S) dtdx, ph.scheme);
   const int cb = blockIdx.x % ncolblk;
-  const int sgi = (blockIdx.x / ncolblk) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int sgi = (blockIdx.x / ncolblk) * (blockDim.x >> 6) + wave_id();
   const int col = cb * 64 + (threadIdx.x & 63);
   const bool colok = col < g.nx;
   const long i = 2 + (colok ? col : g.nx - 1);
@@ -200,14 +200,19 @@ void launch_sweep(int idim, const GridGeom& g, const PhysArgs& pa, GridView<S> i
     else { if (c) HX(false, true); else HX(false, false); }
 #undef HX
   } else {
-    const int yseg = tune.yseg > 0 ? tune.yseg : 64;
+    int yseg = tune.yseg > 0 ? tune.yseg : 64;
     const int yw = tune.ywaves > 0 ? tune.ywaves : 1;
     const int ncolblk = (int) cdiv(g.nx, 64);
-    const int nsg = (int) cdiv(g.ny, (long) yseg * yw);
-    const dim3 grid((unsigned) ncolblk * nsg), blk(64 * yw);
-#define HY(A, B) \
-  hipLaunchKernelGGL((k_sweep_y<S, A, B>), grid, blk, 0, st, g, ph, pa.dx, dt_val, ctl, in, out, \
-                     cour, yseg, ncolblk)
+    // a narrow or short grid has few columns to march: shorten the segments
+    // (down to 16 rows; each adds a 4-row halo) until there are about two
+    // waves per SIMD, rather than leave most of the GPU idle
+    while (yseg > 16 && (long) ncolblk * cdiv(g.ny, yseg) < 2048) yseg >>= 1;
+#define HY(A, B)                                                                               \
+  do {                                                                                         \
+    const int nsg = (int) cdiv(g.ny, (long) yseg * yw);                                        \
+    hipLaunchKernelGGL((k_sweep_y<S, A, B>), dim3((unsigned) ncolblk * nsg), dim3(64 * yw), 0, \
+                       st, g, ph, pa.dx, dt_val, ctl, in, out, cour, yseg, ncolblk);           \
+  } while (0)
     if (st1) { if (c) HY(true, true); else HY(true, false); }
     else { if (c) HY(false, true); else HY(false, false); }
 #undef HY

@@ -87,4 +87,9 @@ void launch_init(const GridGeom& g, GridView<S> a, GridView<S> b, int bx, int by
 
 void launch_copy_probe(const double* a, double* b, long n, hipStream_t st);
 
+// Pitched device rows -> contiguous host rows through pinned staging buffers
+// (hydro_stage.cpp); dst_esz == src_esz, or 8 with src_esz == 4 (widening).
+int stage_d2h(void* dst, int dst_esz, const void* src, int src_esz, long src_pitch, long width,
+              long rows, hipStream_t st);
+
 }  // namespace hydro
