@@ -63,6 +63,7 @@ struct hydro_amd_batch {
   int* cids = nullptr;           // ids needing a CFL pass (device)
   int n[6] = {0, 0, 0, 0, 0, 0}; // Px Py Qx Qy PB QB
   int off[6] = {0, 0, 0, 0, 0, 0};
+  int n1[6] = {0, 0, 0, 0, 0, 0};  // leading tasks of slope_type == 1 instances
   int nids = 0;
   int maxid = -1;                // highest live id
   bool dirty = true;
@@ -400,16 +401,21 @@ static int rebuild(hydro_amd_batch* b) {
       b->maxid = k;
     }
   if (read_ctl(b)) return -1;
-  std::vector<BTask> L[6];
+  // each sweep list: tasks of slope_type == 1 instances first (their own
+  // kernel variant), then the others
+  std::vector<BTask> L[6], G[6];
   for (int id : b->live_ids) {
     const bool even = (b->hctl[id].nstep % 2) == 0;
-    tasks_of(b, id, 1, even ? L[0] : L[2]);
-    tasks_of(b, id, 2, even ? L[3] : L[1]);
+    std::vector<BTask>* S = b->inst[id].p.slope_type == 1.0 ? L : G;
+    tasks_of(b, id, 1, even ? S[0] : S[2]);
+    tasks_of(b, id, 2, even ? S[3] : S[1]);
     boundary_tasks_of(b, id, even ? 1 : 2, L[4]);
     boundary_tasks_of(b, id, even ? 2 : 1, L[5]);
   }
   int need = 0;
   for (int k = 0; k < 6; k++) {
+    b->n1[k] = (int) L[k].size();
+    L[k].insert(L[k].end(), G[k].begin(), G[k].end());
     b->off[k] = need;
     b->n[k] = (int) L[k].size();
     need += b->n[k];
@@ -452,8 +458,10 @@ static void enq_phase(hydro_amd_batch* b, int which, int phase) {
   const InstDev<S>* I = (const InstDev<S>*) b->idev;
   const int x = which == 0 ? 0 : 2, y = x + 1, bd = which == 0 ? 4 : 5;
   launch_batch_boundary<S>(I, b->tasks + b->off[bd], b->n[bd], phase, b->nblk_y, b->st);
-  launch_batch_sweep<S>(1, I, b->tasks + b->off[x], b->n[x], phase, b->tune, b->nblk_x, b->st);
-  launch_batch_sweep<S>(2, I, b->tasks + b->off[y], b->n[y], phase, b->tune, b->nblk_y, b->st);
+  launch_batch_sweep<S>(1, I, b->tasks + b->off[x], b->n[x], b->n1[x], phase, b->tune, b->nblk_x,
+                        b->st);
+  launch_batch_sweep<S>(2, I, b->tasks + b->off[y], b->n[y], b->n1[y], phase, b->tune, b->nblk_y,
+                        b->st);
 }
 
 template <typename S>
@@ -619,10 +627,12 @@ extern "C" int hydro_amd_batch_autotune(hydro_amd_batch* b, int reps, int* best)
             const int nb = kind == 1 ? b->nblk_x : b->nblk_y;
             if (b->prec == HYDRO_AMD_FP64)
               launch_batch_sweep<double>(kind, (const InstDev<double>*) b->idev,
-                                         b->tasks + b->off[k], b->n[k], 0, b->tune, nb, b->st);
+                                         b->tasks + b->off[k], b->n[k], b->n1[k], 0, b->tune,
+                                         nb, b->st);
             else
               launch_batch_sweep<float>(kind, (const InstDev<float>*) b->idev,
-                                        b->tasks + b->off[k], b->n[k], 0, b->tune, nb, b->st);
+                                        b->tasks + b->off[k], b->n[k], b->n1[k], 0, b->tune, nb,
+                                        b->st);
           }
         }
         (void) hipEventRecord(e1, b->st);

@@ -18,9 +18,12 @@
 
 namespace hydro {
 
-// x tasks of one phase (every task's kind is 1).
-template <typename S>
-__global__ __launch_bounds__(256) void k_batch_sweep_x(const InstDev<S>* inst, const BTask* tasks,
+// x tasks of one phase (every task's kind is 1).  ST1: every listed
+// instance has slope_type == 1 (the host splits each phase's lists), so the
+// limiter's multiply/divide by slope_type is compiled out as in single grids.
+template <typename S, bool ST1>
+__global__ __launch_bounds__(256) void k_batch_sweep_x(const InstDev<S>* __restrict__ inst,
+                                                       const BTask* __restrict__ tasks,
                                                        int ntask, int phase, int xiter) {
   const int wpb = blockDim.x >> 6;
   const int wave = wave_id();
@@ -34,18 +37,12 @@ __global__ __launch_bounds__(256) void k_batch_sweep_x(const InstDev<S>* inst, c
     // the second sweep of an odd step is an x sweep; it produces the CFL
     // maxima the next (even) step's dt needs
     const bool cour = phase == 1;
-    const bool st1 = d.ph.slope_type == (S) 1;
     double cx = 0.0, cy = 0.0;
     // segments tk.a + it * wpb + wave, it = 0 .. xiter-1
     SegWalk w(tk.a + wave, d.nseg);
     for (int it = 0; it < xiter && w.row < d.g.ny; it++) {
-      if (st1) {
-        if (cour) x_segment<S, true, true>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx, cy);
-        else x_segment<S, true, false>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx, cy);
-      } else {
-        if (cour) x_segment<S, false, true>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx, cy);
-        else x_segment<S, false, false>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx, cy);
-      }
+      if (cour) x_segment<S, ST1, true>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx, cy);
+      else x_segment<S, ST1, false>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx, cy);
       w.advance(wpb, d.nseg);
     }
     if (cour) commit_courant(cx, cy, d.cour, kInstSlots, t % kInstSlots);
@@ -54,8 +51,9 @@ __global__ __launch_bounds__(256) void k_batch_sweep_x(const InstDev<S>* inst, c
 
 // y tasks of one phase (every task's kind is 2); y sweeps never end a step
 // whose successor needs dt, so they fold no CFL maxima.
-template <typename S>
-__global__ __launch_bounds__(256) void k_batch_sweep_y(const InstDev<S>* inst, const BTask* tasks,
+template <typename S, bool ST1>
+__global__ __launch_bounds__(256) void k_batch_sweep_y(const InstDev<S>* __restrict__ inst,
+                                                       const BTask* __restrict__ tasks,
                                                        int ntask, int phase, int yseg) {
   const int wpb = blockDim.x >> 6;
   const int wave = wave_id();
@@ -73,10 +71,7 @@ __global__ __launch_bounds__(256) void k_batch_sweep_y(const InstDev<S>* inst, c
     double cx = 0.0, cy = 0.0;
     if (j0 < d.g.ny + 2) {  // wave-uniform
       const int j1 = min(j0 + yseg, d.g.ny + 2);
-      if (d.ph.slope_type == (S) 1)
-        y_march<S, true, false>(d.g, d.ph, tr, dtdx, in, out, colok, i, j0, j1, cx, cy);
-      else
-        y_march<S, false, false>(d.g, d.ph, tr, dtdx, in, out, colok, i, j0, j1, cx, cy);
+      y_march<S, ST1, false>(d.g, d.ph, tr, dtdx, in, out, colok, i, j0, j1, cx, cy);
     }
   }
 }
@@ -202,25 +197,33 @@ template <typename S>
 int batch_sweep_occupancy(int kind, int waves) {
   int nb = 0;
   const hipError_t e =
-      kind == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_sweep_x<S>, 64 * waves, 0)
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_sweep_y<S>, 64 * waves, 0);
+      kind == 1
+          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_sweep_x<S, true>, 64 * waves, 0)
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_sweep_y<S, true>, 64 * waves, 0);
   if (e != hipSuccess) return 2;
   return nb > 0 ? nb : 1;
 }
 template int batch_sweep_occupancy<double>(int, int);
 template int batch_sweep_occupancy<float>(int, int);
 
-template <typename S>
-void launch_batch_sweep(int kind, const InstDev<S>* inst, const BTask* tasks, int ntask,
-                        int phase, const BatchTune& tn, int nblocks, hipStream_t st) {
+template <typename S, bool ST1>
+static void launch_sweep_part(int kind, const InstDev<S>* inst, const BTask* tasks, int ntask,
+                              int phase, const BatchTune& tn, int nblocks, hipStream_t st) {
   if (ntask <= 0) return;
   const int nb = ntask < nblocks ? ntask : nblocks;
   if (kind == 1)
-    hipLaunchKernelGGL(k_batch_sweep_x<S>, dim3(nb), dim3(64 * tn.waves), 0, st, inst, tasks,
-                       ntask, phase, tn.xiter);
+    hipLaunchKernelGGL((k_batch_sweep_x<S, ST1>), dim3(nb), dim3(64 * tn.waves), 0, st, inst,
+                       tasks, ntask, phase, tn.xiter);
   else
-    hipLaunchKernelGGL(k_batch_sweep_y<S>, dim3(nb), dim3(64 * tn.waves), 0, st, inst, tasks,
-                       ntask, phase, tn.yseg);
+    hipLaunchKernelGGL((k_batch_sweep_y<S, ST1>), dim3(nb), dim3(64 * tn.waves), 0, st, inst,
+                       tasks, ntask, phase, tn.yseg);
+}
+
+template <typename S>
+void launch_batch_sweep(int kind, const InstDev<S>* inst, const BTask* tasks, int ntask,
+                        int nst1, int phase, const BatchTune& tn, int nblocks, hipStream_t st) {
+  launch_sweep_part<S, true>(kind, inst, tasks, nst1, phase, tn, nblocks, st);
+  launch_sweep_part<S, false>(kind, inst, tasks + nst1, ntask - nst1, phase, tn, nblocks, st);
 }
 
 template <typename S>
@@ -253,7 +256,7 @@ void launch_batch_ctl(const InstDev<S>* inst, const int* ids, int nids, bool beg
 }
 
 #define INST(S)                                                                                  \
-  template void launch_batch_sweep<S>(int, const InstDev<S>*, const BTask*, int, int,           \
+  template void launch_batch_sweep<S>(int, const InstDev<S>*, const BTask*, int, int, int,      \
                                       const BatchTune&, int, hipStream_t);                        \
   template void launch_batch_boundary<S>(const InstDev<S>*, const BTask*, int, int, int,        \
                                          hipStream_t);                                           \

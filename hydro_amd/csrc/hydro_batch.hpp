@@ -34,10 +34,11 @@ struct BatchTune {
   int yseg;   // rows each wave marches in a y task
 };
 
-// kind 1: x tasks, 2: y tasks (a phase launches both, on their own tables)
+// kind 1: x tasks, 2: y tasks (a phase launches both, on their own tables).
+// The first nst1 tasks belong to instances with slope_type == 1.
 template <typename S>
 void launch_batch_sweep(int kind, const InstDev<S>* inst, const BTask* tasks, int ntask,
-                        int phase, const BatchTune& tn, int nblocks, hipStream_t st);
+                        int nst1, int phase, const BatchTune& tn, int nblocks, hipStream_t st);
 template <typename S>
 void launch_batch_boundary(const InstDev<S>* inst, const BTask* tasks, int ntask, int phase,
                            int nblocks, hipStream_t st);

@@ -122,6 +122,22 @@ def test_instances_stop_at_their_own_limits():
         assert [q[i][3] for i in ids] == [0, 0, 0, 1]
 
 
+def test_batch_mixed_slope_types():
+    """Instances with slope_type 1 and others share a phase (their tasks run in
+    two kernel variants); each stays identical to its single-grid run."""
+    insts = make_instances(7, 6)
+    for k, (p, *_rest) in enumerate(insts):
+        p.slope_type = (1.0, 2.0, 0.5)[k % 3]
+    with hydro_amd.Batch() as b:
+        ids = [b.add(p) for p, *_ in insts]
+        for i, (p, bx, by, e) in zip(ids, insts):
+            b.init_blast(i, bx, by, e)
+        b.run(13)
+        for i, (p, bx, by, e) in zip(ids, insts):
+            u_ref, _, _ = alone(p, bx, by, e, 13)
+            check_same(b.download(i), u_ref, f"slope_type {p.slope_type} instance {i}")
+
+
 def test_budget_admission():
     with hydro_amd.Batch(budget_bytes=0) as probe:
         need = probe.bytes_needed(100, 100)
