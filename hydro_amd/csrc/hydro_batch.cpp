@@ -64,6 +64,7 @@ struct hydro_amd_batch {
   int n[6] = {0, 0, 0, 0, 0, 0}; // Px Py Qx Qy PB QB
   int off[6] = {0, 0, 0, 0, 0, 0};
   int n1[6] = {0, 0, 0, 0, 0, 0};  // leading tasks of slope_type == 1 instances
+  uint64_t* qctr = nullptr;        // kQueues task-queue counters (device)
   int nids = 0;
   int maxid = -1;                // highest live id
   bool dirty = true;
@@ -103,7 +104,7 @@ extern "C" void hydro_amd_batch_destroy(hydro_amd_batch* b) {
     for (auto& m : in.mem)
       if (m) (void) hipFree(m);
   for (void* p : {(void*) b->ctl, (void*) b->cour, b->idev, (void*) b->tasks, (void*) b->ids,
-                  (void*) b->cids})
+                  (void*) b->cids, (void*) b->qctr})
     if (p) (void) hipFree(p);
   if (b->hctl) (void) hipHostFree(b->hctl);
   if (b->hidev) (void) hipHostFree(b->hidev);
@@ -128,6 +129,7 @@ extern "C" hydro_amd_batch* hydro_amd_batch_create(int precision, long long budg
             hipMalloc(&b->idev, isz * kMaxInst) == hipSuccess &&
             hipMalloc(&b->ids, sizeof(int) * kMaxInst) == hipSuccess &&
             hipMalloc(&b->cids, sizeof(int) * kMaxInst) == hipSuccess &&
+            hipMalloc(&b->qctr, sizeof(uint64_t) * kQueues) == hipSuccess &&
             hipHostMalloc((void**) &b->hctl, sizeof(RunCtl) * kMaxInst) == hipSuccess &&
             hipHostMalloc(&b->hidev, isz * kMaxInst) == hipSuccess;
   if (!ok) {
@@ -458,19 +460,20 @@ static void enq_phase(hydro_amd_batch* b, int which, int phase) {
   const InstDev<S>* I = (const InstDev<S>*) b->idev;
   const int x = which == 0 ? 0 : 2, y = x + 1, bd = which == 0 ? 4 : 5;
   launch_batch_boundary<S>(I, b->tasks + b->off[bd], b->n[bd], phase, b->nblk_y, b->st);
+  uint64_t* q = b->qctr + 4 * phase;  // x: q[0..1], y: q[2..3]
   launch_batch_sweep<S>(1, I, b->tasks + b->off[x], b->n[x], b->n1[x], phase, b->tune, b->nblk_x,
-                        b->st);
+                        q, b->st);
   launch_batch_sweep<S>(2, I, b->tasks + b->off[y], b->n[y], b->n1[y], phase, b->tune, b->nblk_y,
-                        b->st);
+                        q + 2, b->st);
 }
 
 template <typename S>
 static void enq_step_t(hydro_amd_batch* b, bool flip) {
   const InstDev<S>* I = (const InstDev<S>*) b->idev;
-  launch_batch_ctl<S>(I, b->ids, b->nids, true, b->st);
+  launch_batch_ctl<S>(I, b->ids, b->nids, true, b->qctr, b->st);
   enq_phase<S>(b, flip ? 1 : 0, 0);
   enq_phase<S>(b, flip ? 0 : 1, 1);
-  launch_batch_ctl<S>(I, b->ids, b->nids, false, b->st);
+  launch_batch_ctl<S>(I, b->ids, b->nids, false, b->qctr, b->st);
 }
 
 static void enq_step(hydro_amd_batch* b, bool flip) {
@@ -622,17 +625,18 @@ extern "C" int hydro_amd_batch_autotune(hydro_amd_batch* b, int reps, int* best)
         for (int r = -1; r < reps; r++) {  // r = -1 warms up
           if (r == 0) (void) hipEventRecord(e0, b->st);
           // every x and y list at phase 0: reads buffer 0, writes buffer 1 only
+          launch_zero_u64(b->qctr, kQueues, b->st);
           for (int k = 0; k < 4; k++) {
             const int kind = (k % 2 == 0) ? 1 : 2;
             const int nb = kind == 1 ? b->nblk_x : b->nblk_y;
             if (b->prec == HYDRO_AMD_FP64)
               launch_batch_sweep<double>(kind, (const InstDev<double>*) b->idev,
                                          b->tasks + b->off[k], b->n[k], b->n1[k], 0, b->tune,
-                                         nb, b->st);
+                                         nb, b->qctr + 2 * k, b->st);
             else
               launch_batch_sweep<float>(kind, (const InstDev<float>*) b->idev,
                                         b->tasks + b->off[k], b->n[k], b->n1[k], 0, b->tune, nb,
-                                        b->st);
+                                        b->qctr + 2 * k, b->st);
           }
         }
         (void) hipEventRecord(e1, b->st);

@@ -18,16 +18,33 @@
 
 namespace hydro {
 
+// The next task index of a sweep launch for the whole workgroup: persistent
+// workgroups pull tasks from one counter (zeroed before the launch), so a
+// workgroup that drew cheap tasks takes more instead of idling while others
+// finish.  Every index past the table ends the loop of the workgroup that
+// drew it, so the launch drains.
+__device__ __forceinline__ int next_task(uint64_t* qnext) {
+  __shared__ int s_next;
+  if (threadIdx.x == 0) s_next = (int) atomicAdd((unsigned long long*) qnext, 1ULL);
+  __syncthreads();
+  const int t = s_next;
+  __syncthreads();  // every thread has read s_next before it is rewritten
+  return t;
+}
+
 // x tasks of one phase (every task's kind is 1).  ST1: every listed
 // instance has slope_type == 1 (the host splits each phase's lists), so the
 // limiter's multiply/divide by slope_type is compiled out as in single grids.
 template <typename S, bool ST1>
 __global__ __launch_bounds__(256) void k_batch_sweep_x(const InstDev<S>* __restrict__ inst,
                                                        const BTask* __restrict__ tasks,
-                                                       int ntask, int phase, int xiter) {
+                                                       int ntask, int phase, int xiter,
+                                                       uint64_t* qnext) {
   const int wpb = blockDim.x >> 6;
   const int wave = wave_id();
-  for (int t = blockIdx.x; t < ntask; t += gridDim.x) {
+  for (;;) {
+    const int t = next_task(qnext);
+    if (t >= ntask) break;
     const BTask tk = tasks[t];
     const InstDev<S>& d = inst[tk.inst];
     if (!d.ctl->active) continue;  // workgroup-uniform
@@ -54,10 +71,13 @@ __global__ __launch_bounds__(256) void k_batch_sweep_x(const InstDev<S>* __restr
 template <typename S, bool ST1>
 __global__ __launch_bounds__(256) void k_batch_sweep_y(const InstDev<S>* __restrict__ inst,
                                                        const BTask* __restrict__ tasks,
-                                                       int ntask, int phase, int yseg) {
+                                                       int ntask, int phase, int yseg,
+                                                       uint64_t* qnext) {
   const int wpb = blockDim.x >> 6;
   const int wave = wave_id();
-  for (int t = blockIdx.x; t < ntask; t += gridDim.x) {
+  for (;;) {
+    const int t = next_task(qnext);
+    if (t >= ntask) break;
     const BTask tk = tasks[t];
     const InstDev<S>& d = inst[tk.inst];
     if (!d.ctl->active) continue;
@@ -125,7 +145,9 @@ __global__ __launch_bounds__(256) void k_batch_courant(const InstDev<S>* inst, c
 // broken down) and flags it, so one bad run cannot stall the others.
 template <typename S>
 __global__ __launch_bounds__(64) void k_batch_ctl_begin(const InstDev<S>* inst, const int* ids,
-                                                        int nids) {
+                                                        int nids, uint64_t* qctr) {
+  // the step's sweep launches start their task queues from 0
+  if (blockIdx.x == 0 && threadIdx.x < kQueues) qctr[threadIdx.x] = 0;
   for (int k = blockIdx.x; k < nids; k += gridDim.x) {
     const InstDev<S>& d = inst[ids[k]];
     RunCtl* c = d.ctl;
@@ -208,22 +230,25 @@ template int batch_sweep_occupancy<float>(int, int);
 
 template <typename S, bool ST1>
 static void launch_sweep_part(int kind, const InstDev<S>* inst, const BTask* tasks, int ntask,
-                              int phase, const BatchTune& tn, int nblocks, hipStream_t st) {
+                              int phase, const BatchTune& tn, int nblocks, uint64_t* q,
+                              hipStream_t st) {
   if (ntask <= 0) return;
   const int nb = ntask < nblocks ? ntask : nblocks;
   if (kind == 1)
     hipLaunchKernelGGL((k_batch_sweep_x<S, ST1>), dim3(nb), dim3(64 * tn.waves), 0, st, inst,
-                       tasks, ntask, phase, tn.xiter);
+                       tasks, ntask, phase, tn.xiter, q);
   else
     hipLaunchKernelGGL((k_batch_sweep_y<S, ST1>), dim3(nb), dim3(64 * tn.waves), 0, st, inst,
-                       tasks, ntask, phase, tn.yseg);
+                       tasks, ntask, phase, tn.yseg, q);
 }
 
 template <typename S>
 void launch_batch_sweep(int kind, const InstDev<S>* inst, const BTask* tasks, int ntask,
-                        int nst1, int phase, const BatchTune& tn, int nblocks, hipStream_t st) {
-  launch_sweep_part<S, true>(kind, inst, tasks, nst1, phase, tn, nblocks, st);
-  launch_sweep_part<S, false>(kind, inst, tasks + nst1, ntask - nst1, phase, tn, nblocks, st);
+                        int nst1, int phase, const BatchTune& tn, int nblocks, uint64_t* q,
+                        hipStream_t st) {
+  launch_sweep_part<S, true>(kind, inst, tasks, nst1, phase, tn, nblocks, q, st);
+  launch_sweep_part<S, false>(kind, inst, tasks + nst1, ntask - nst1, phase, tn, nblocks, q + 1,
+                              st);
 }
 
 template <typename S>
@@ -244,11 +269,11 @@ void launch_batch_courant(const InstDev<S>* inst, const int* ids, int nids, hipS
 
 template <typename S>
 void launch_batch_ctl(const InstDev<S>* inst, const int* ids, int nids, bool begin,
-                      hipStream_t st) {
+                      uint64_t* qctr, hipStream_t st) {
   if (nids <= 0) return;
   if (begin) {
     const int nb = nids < 1024 ? nids : 1024;
-    hipLaunchKernelGGL(k_batch_ctl_begin<S>, dim3(nb), dim3(64), 0, st, inst, ids, nids);
+    hipLaunchKernelGGL(k_batch_ctl_begin<S>, dim3(nb), dim3(64), 0, st, inst, ids, nids, qctr);
   } else {
     hipLaunchKernelGGL(k_batch_ctl_end<S>, dim3(cdivu(nids, 256)), dim3(256), 0, st, inst, ids,
                        nids);
@@ -257,11 +282,12 @@ void launch_batch_ctl(const InstDev<S>* inst, const int* ids, int nids, bool beg
 
 #define INST(S)                                                                                  \
   template void launch_batch_sweep<S>(int, const InstDev<S>*, const BTask*, int, int, int,      \
-                                      const BatchTune&, int, hipStream_t);                        \
+                                      const BatchTune&, int, uint64_t*, hipStream_t);             \
   template void launch_batch_boundary<S>(const InstDev<S>*, const BTask*, int, int, int,        \
                                          hipStream_t);                                           \
   template void launch_batch_courant<S>(const InstDev<S>*, const int*, int, hipStream_t);       \
-  template void launch_batch_ctl<S>(const InstDev<S>*, const int*, int, bool, hipStream_t);
+  template void launch_batch_ctl<S>(const InstDev<S>*, const int*, int, bool, uint64_t*,      \
+                                    hipStream_t);
 INST(double)
 INST(float)
 #undef INST

@@ -7,6 +7,8 @@ namespace hydro {
 
 constexpr int kInstSlots = 64;        // CFL max slots per instance
 constexpr int kBoundaryChunk = 1024;  // (line, var) ghost fills per boundary task
+// Task-queue counters: [phase][x, y][slope_type == 1 variant, other]
+constexpr int kQueues = 8;
 
 // One instance as the batched kernels see it (device array, rebuilt by the
 // host whenever the set of instances changes).
@@ -35,10 +37,12 @@ struct BatchTune {
 };
 
 // kind 1: x tasks, 2: y tasks (a phase launches both, on their own tables).
-// The first nst1 tasks belong to instances with slope_type == 1.
+// The first nst1 tasks belong to instances with slope_type == 1.  q: the two
+// task-queue counters of this list (zero at launch).
 template <typename S>
 void launch_batch_sweep(int kind, const InstDev<S>* inst, const BTask* tasks, int ntask,
-                        int nst1, int phase, const BatchTune& tn, int nblocks, hipStream_t st);
+                        int nst1, int phase, const BatchTune& tn, int nblocks, uint64_t* q,
+                        hipStream_t st);
 template <typename S>
 void launch_batch_boundary(const InstDev<S>* inst, const BTask* tasks, int ntask, int phase,
                            int nblocks, hipStream_t st);
@@ -46,7 +50,7 @@ template <typename S>
 void launch_batch_courant(const InstDev<S>* inst, const int* ids, int nids, hipStream_t st);
 template <typename S>
 void launch_batch_ctl(const InstDev<S>* inst, const int* ids, int nids, bool begin,
-                      hipStream_t st);
+                      uint64_t* qctr, hipStream_t st);  // begin also zeroes qctr[0..kQueues)
 
 void launch_batch_set_ctl(RunCtl* c, double t, double dt, double tend, int nstep, int nstepmax,
                           int active, hipStream_t st);
