@@ -121,6 +121,8 @@ def _serve(runs, chunk):
     import hydro_amd
     srv = hydro_amd.Server(precision=hydro_amd.MIXED, budget_bytes=200 * 10**9,
                            chunk_steps=chunk, keep_results=True)
+    # pinned room for every result (FP32 state incl. ghosts, 4 variables)
+    srv.reserve_results(sum(16 * (r.params.nx + 4) * (r.params.ny + 4) + 256 for r in runs))
     t_start = hydro_amd.now()
     ids, k, n = [], 0, len(runs)
     while True:

@@ -190,7 +190,8 @@ def lib() -> C.CDLL:
                        ("hydro_amd_server_info", [S, C.c_int, C.c_void_p, C.c_void_p,
                                                   C.c_void_p]),
                        ("hydro_amd_server_result", [S, C.c_int, _dp]),
-                       ("hydro_amd_server_release", [S, C.c_int])]:
+                       ("hydro_amd_server_release", [S, C.c_int]),
+                       ("hydro_amd_server_reserve_results", [S, C.c_longlong])]:
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
@@ -495,3 +496,7 @@ class Server:
 
     def release(self, r: int) -> None:
         _check(lib().hydro_amd_server_release(self._h, r))
+
+    def reserve_results(self, nbytes: int) -> None:
+        """Pin host memory for kept results (copied by DMA beside the kernels)."""
+        _check(lib().hydro_amd_server_reserve_results(self._h, int(nbytes)))

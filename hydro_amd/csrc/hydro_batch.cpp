@@ -70,7 +70,7 @@ struct hydro_amd_batch {
   bool dirty = true;
   bool flip = false;             // next step runs Q before P
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  BatchTune tune = {4, 4, 32};
+  BatchTune tune = {2, 8, 32};  // what the autotuner picks for the batched and serving mixes
   int nblk_x = 1024, nblk_y = 1024, cus = 256;
   std::vector<int> live_ids;
   RunCtl* hctl = nullptr;        // pinned host copy of the run controls
@@ -294,15 +294,32 @@ extern "C" int hydro_amd_batch_download(hydro_amd_batch* b, int id, double* u) {
   return stage_d2h(u, 8, src, (int) b->esz, in->g.pitch, nxt, rows, b->st) ? -1 : 0;
 }
 
+// Where an instance's state lives: `rows` rows of `width` elements of `esz`
+// bytes, `pitch` elements apart, starting at *src (layout [4][ny+4][nx+4]).
+int hydro_amd_batch_raw_view(hydro_amd_batch* b, int id, const void** src, long* pitch,
+                             long* width, long* rows, int* esz) {
+  Inst* in = live(b, id);
+  if (!in) return -1;
+  *src = (const char*) in->mem[0] + kOrigin * b->esz;
+  *pitch = in->g.pitch;
+  *width = in->p.nx + 4;
+  *rows = 4L * (in->p.ny + 4);
+  *esz = (int) b->esz;
+  return 0;
+}
+
 // Copies an instance's state as stored (element size 8 or 4), layout
 // [4][ny+4][nx+4]; for the serving loop (no conversion on its critical path).
-int hydro_amd_batch_download_raw(hydro_amd_batch* b, int id, void* dst, int* elem_size) {
+// The copy is queued on `st` (the batch's stream if null), so it can run
+// beside kernels on the batch's stream that do not touch this instance.
+int hydro_amd_batch_download_raw(hydro_amd_batch* b, int id, void* dst, int* elem_size,
+                                 hipStream_t st) {
   Inst* in = live(b, id);
   if (!in || !dst) return in ? hydro_amd_set_error("null argument") : -1;
   const long nxt = in->p.nx + 4, rows = 4L * (in->p.ny + 4);
   const size_t w = b->esz;
   const void* src = (const char*) in->mem[0] + kOrigin * w;
-  if (stage_d2h(dst, (int) w, src, (int) w, in->g.pitch, nxt, rows, b->st)) return -1;
+  if (stage_d2h(dst, (int) w, src, (int) w, in->g.pitch, nxt, rows, st ? st : b->st)) return -1;
   if (elem_size) *elem_size = (int) w;
   return 0;
 }

@@ -3,11 +3,13 @@
 //
 // Continuous batching: requests (independent hydro runs) queue in arrival
 // order; each pump admits queued requests while they fit the HBM budget,
-// advances every running request by `chunk` steps in one batched device run
-// (new arrivals join the runs already on the GPU), then retires the ones whose
-// time loop has ended (t >= tend or nstep >= nstepmax): results are copied out
-// and their memory returns to the budget.  A request whose dt stops being a
-// positive finite number is failed and retired on its own; the others go on.
+// queues `chunk` steps of every running request as one batched device run
+// (new arrivals join the runs already on the GPU), and while that runs copies
+// out the results of the requests whose time loop ended (t >= tend or nstep
+// >= nstepmax) in the previous pump -- on a second stream, so the copies
+// overlap the kernels -- and returns their memory to the budget.  A request
+// whose dt stops being a positive finite number is failed and retired on its
+// own; the others go on.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -15,15 +17,58 @@
 
 #include <deque>
 #include <map>
+#include <memory>
+#include <utility>
 #include <vector>
 
 #include "../../include/hydro_amd.h"
 
 int hydro_amd_set_error(const char* fmt, ...);  // hydro_capi.cpp
-int hydro_amd_batch_download_raw(hydro_amd_batch* b, int id, void* dst, int* elem_size);
+int hydro_amd_batch_download_raw(hydro_amd_batch* b, int id, void* dst, int* elem_size,
+                                 hipStream_t st);  // hydro_batch.cpp
+int hydro_amd_batch_raw_view(hydro_amd_batch* b, int id, const void** src, long* pitch,
+                             long* width, long* rows, int* esz);
 
 namespace {
 enum { QUEUED = 0, RUNNING = 1, DONE = 2, FAILED = 3 };
+
+// Pinned host memory for results: first-fit blocks (256-byte aligned) with
+// coalescing frees.  A result placed here is copied by DMA straight from the
+// GPU, asynchronously, beside the kernels.
+struct Arena {
+  unsigned char* base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> free_;  // offset -> length
+  bool alloc(size_t n, size_t* off) {
+    n = (n + 255) & ~(size_t) 255;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < n) continue;
+      *off = it->first;
+      const size_t rest = it->second - n;
+      const size_t at = it->first + n;
+      free_.erase(it);
+      if (rest) free_[at] = rest;
+      return true;
+    }
+    return false;
+  }
+  void release(size_t off, size_t n) {
+    n = (n + 255) & ~(size_t) 255;
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
 
 struct Request {
   hydro_amd_params p;
@@ -34,8 +79,13 @@ struct Request {
   int inst = -1;
   double t_sim = 0, dt = 0;
   int nstep = 0;
-  std::vector<unsigned char> result;  // state as stored (8- or 4-byte elements)
+  std::unique_ptr<unsigned char[]> result;  // state as stored (8- or 4-byte elements)
+  size_t result_bytes = 0;
+  size_t arena_off = ~(size_t) 0;           // or here, in the server's pinned arena
   int esz = 0;
+  const unsigned char* data(const Arena& a) const {
+    return arena_off != ~(size_t) 0 ? a.base + arena_off : result.get();
+  }
 };
 
 double now_s() {
@@ -47,11 +97,15 @@ double now_s() {
 
 struct hydro_amd_server {
   hydro_amd_batch* b = nullptr;
+  hipStream_t copy_st = nullptr;  // result copies, beside the batch's kernels
   int chunk = 20;
   int keep = 0;
   std::vector<Request> req;
   std::deque<int> queue;
-  std::map<int, int> running;  // instance id -> request id
+  std::map<int, int> running;   // instance id -> request id
+  std::map<int, int> finished;  // ended on the device, result not yet copied out
+  std::vector<std::pair<int, int>> copying;  // (instance, request): DMA into the arena queued
+  Arena arena;
   int ndone = 0, nfailed = 0;
 };
 
@@ -67,6 +121,12 @@ extern "C" hydro_amd_server* hydro_amd_server_create(int precision, long long bu
   if (!b) return nullptr;
   hydro_amd_server* s = new hydro_amd_server();
   s->b = b;
+  if (hipStreamCreateWithFlags(&s->copy_st, hipStreamNonBlocking) != hipSuccess) {
+    hydro_amd_set_error("stream creation failed");
+    hydro_amd_batch_destroy(b);
+    delete s;
+    return nullptr;
+  }
   s->chunk = chunk_steps;
   s->keep = keep_results;
   return s;
@@ -74,8 +134,24 @@ extern "C" hydro_amd_server* hydro_amd_server_create(int precision, long long bu
 
 extern "C" void hydro_amd_server_destroy(hydro_amd_server* s) {
   if (!s) return;
+  if (s->copy_st) (void) hipStreamSynchronize(s->copy_st);
   hydro_amd_batch_destroy(s->b);
+  if (s->copy_st) (void) hipStreamDestroy(s->copy_st);
+  if (s->arena.base) (void) hipHostFree(s->arena.base);
   delete s;
+}
+
+extern "C" int hydro_amd_server_reserve_results(hydro_amd_server* s, long long bytes) {
+  if (!s || bytes < 0) return hydro_amd_set_error("bad argument");
+  if (s->arena.base) return hydro_amd_set_error("result memory already reserved");
+  if (bytes == 0) return 0;
+  void* p = nullptr;
+  if (hipHostMalloc(&p, (size_t) bytes, hipHostMallocDefault) != hipSuccess)
+    return hydro_amd_set_error("pinned allocation of %lld bytes failed", bytes);
+  s->arena.base = (unsigned char*) p;
+  s->arena.size = (size_t) bytes & ~(size_t) 255;
+  s->arena.free_[0] = s->arena.size;
+  return 0;
 }
 
 extern "C" hydro_amd_batch* hydro_amd_server_batch(hydro_amd_server* s) { return s ? s->b : nullptr; }
@@ -91,18 +167,49 @@ extern "C" int hydro_amd_server_submit(hydro_amd_server* s, const hydro_amd_para
   r.by = by;
   r.e = e;
   r.t_submit = now_s();
-  s->req.push_back(r);
+  s->req.push_back(std::move(r));
   const int id = (int) s->req.size() - 1;
   s->queue.push_back(id);
   return id;
 }
 
+// Queues the DMA of a finished request's result into the pinned arena, if it
+// has room; returns 1 if queued (the request completes when the copy stream
+// has drained), 0 if not.  The instance is inactive on the device (its run
+// control ended), so kernels queued meanwhile on the batch's stream leave its
+// state alone.
+static int start_copy(hydro_amd_server* s, int inst, int rid) {
+  Request& r = s->req[rid];
+  const void* src;
+  long pitch, width, rows;
+  int esz;
+  if (!s->arena.base || hydro_amd_batch_raw_view(s->b, inst, &src, &pitch, &width, &rows, &esz))
+    return 0;
+  const size_t bytes = (size_t) width * rows * esz;
+  size_t off;
+  if (!s->arena.alloc(bytes, &off)) return 0;
+  if (hipMemcpy2DAsync(s->arena.base + off, (size_t) width * esz, src, (size_t) pitch * esz,
+                       (size_t) width * esz, rows, hipMemcpyDeviceToHost, s->copy_st) !=
+      hipSuccess) {
+    s->arena.release(off, bytes);
+    return hydro_amd_set_error("result copy failed");
+  }
+  r.arena_off = off;
+  r.result_bytes = bytes;
+  r.esz = esz;
+  return 1;
+}
+
+// Completes a request: copies its result out (staged, on the copy stream)
+// unless already in the arena, and frees its instance in the batch stream's
+// order.
 static int retire(hydro_amd_server* s, int inst, int rid, int state) {
   Request& r = s->req[rid];
-  if (s->keep && state == DONE) {
+  if (s->keep && state == DONE && r.arena_off == ~(size_t) 0) {
     const size_t n = (size_t) 4 * (r.p.nx + 4) * (r.p.ny + 4);
-    r.result.resize(n * (r.p.precision == HYDRO_AMD_FP64 ? 8 : 4));
-    if (hydro_amd_batch_download_raw(s->b, inst, r.result.data(), &r.esz)) return -1;
+    r.result_bytes = n * (r.p.precision == HYDRO_AMD_FP64 ? 8 : 4);
+    r.result.reset(new unsigned char[r.result_bytes]);
+    if (hydro_amd_batch_download_raw(s->b, inst, r.result.get(), &r.esz, s->copy_st)) return -1;
   }
   if (hydro_amd_batch_remove(s->b, inst)) return -1;
   r.state = state;
@@ -113,17 +220,17 @@ static int retire(hydro_amd_server* s, int inst, int rid, int state) {
   return 0;
 }
 
-// One scheduling round; returns the number of requests still queued or
-// running (0 when the server is idle), or -1 on error.
-extern "C" int hydro_amd_server_pump(hydro_amd_server* s) {
-  if (!s) return hydro_amd_set_error("null server");
-  // admit in arrival order while the head fits
+// One scheduling round; returns the number of requests still queued,
+// running or waiting for their result to be copied out (0 when the server is
+// idle), or -1 on error.
+// Admits queued requests in arrival order while the head fits the budget.
+static int admit(hydro_amd_server* s) {
   while (!s->queue.empty()) {
     const int rid = s->queue.front();
     Request& r = s->req[rid];
     const int inst = hydro_amd_batch_add(s->b, &r.p);
     if (inst < 0) {
-      if (s->running.empty()) {  // can never fit: fail it instead of stalling
+      if (s->running.empty() && s->finished.empty()) {  // can never fit: fail, don't stall
         s->queue.pop_front();
         r.state = FAILED;
         r.t_done = now_s();
@@ -139,8 +246,38 @@ extern "C" int hydro_amd_server_pump(hydro_amd_server* s) {
     r.t_admit = now_s();
     s->running[inst] = rid;
   }
+  return 0;
+}
+
+extern "C" int hydro_amd_server_pump(hydro_amd_server* s) {
+  if (!s) return hydro_amd_set_error("null server");
+  if (admit(s)) return -1;
+  // queue the next chunk, then copy out the runs that ended in the last one
+  // while it executes
+  if (!s->running.empty() && hydro_amd_batch_run_async(s->b, s->chunk)) return -1;
+  bool freed = false;
+  for (const auto& f : s->finished) {
+    const int q = s->keep ? start_copy(s, f.first, f.second) : 0;
+    if (q < 0) return -1;
+    if (q) {
+      s->copying.emplace_back(f.first, f.second);
+    } else {
+      if (retire(s, f.first, f.second, DONE)) return -1;
+      freed = true;
+    }
+  }
+  s->finished.clear();
+  // requests that fit in the memory just returned join from the next chunk
+  if (freed && admit(s)) return -1;
+  if (!s->running.empty() && hydro_amd_batch_sync(s->b)) return -1;
+  if (!s->copying.empty()) {
+    if (hipStreamSynchronize(s->copy_st) != hipSuccess)
+      return hydro_amd_set_error("result copy failed");
+    for (const auto& c : s->copying)
+      if (retire(s, c.first, c.second, DONE)) return -1;
+    s->copying.clear();
+  }
   if (s->running.empty()) return (int) s->queue.size();
-  if (hydro_amd_batch_run(s->b, s->chunk)) return -1;
   const int n = (int) s->running.size();
   std::vector<int> ids(n), nstep(n), more(n);
   std::vector<double> t(n), dt(n);
@@ -156,19 +293,22 @@ extern "C" int hydro_amd_server_pump(hydro_amd_server* s) {
     r.nstep = nstep[k];
     r.dt = dt[k];
     const bool sane = more[k] >= 0 && isfinite(t[k]);
-    if (!sane || more[k] == 0) {
-      if (retire(s, ids[k], rid, sane ? DONE : FAILED)) return -1;
+    if (!sane) {
+      if (retire(s, ids[k], rid, FAILED)) return -1;
+      s->running.erase(it);
+    } else if (more[k] == 0) {
+      s->finished[ids[k]] = rid;
       s->running.erase(it);
     }
   }
-  return (int) (s->queue.size() + s->running.size());
+  return (int) (s->queue.size() + s->running.size() + s->finished.size());
 }
 
 extern "C" int hydro_amd_server_counts(hydro_amd_server* s, int* queued, int* running, int* done,
                                        int* failed) {
   if (!s) return hydro_amd_set_error("null server");
   if (queued) *queued = (int) s->queue.size();
-  if (running) *running = (int) s->running.size();
+  if (running) *running = (int) (s->running.size() + s->finished.size());
   if (done) *done = s->ndone;
   if (failed) *failed = s->nfailed;
   return 0;
@@ -193,13 +333,14 @@ extern "C" int hydro_amd_server_info(hydro_amd_server* s, int rid, double* times
 extern "C" int hydro_amd_server_result(hydro_amd_server* s, int rid, double* uold) {
   if (!s || rid < 0 || rid >= (int) s->req.size()) return hydro_amd_set_error("no request %d", rid);
   Request& r = s->req[rid];
-  if (r.state != DONE || r.result.empty()) return hydro_amd_set_error("no result for %d", rid);
-  const size_t n = r.result.size() / r.esz;
+  const unsigned char* src = r.data(s->arena);
+  if (r.state != DONE || !src) return hydro_amd_set_error("no result for %d", rid);
+  const size_t n = r.result_bytes / r.esz;
   if (r.esz == 8) {
-    const double* d = (const double*) r.result.data();
+    const double* d = (const double*) src;
     for (size_t k = 0; k < n; k++) uold[k] = d[k];
   } else {
-    const float* f = (const float*) r.result.data();
+    const float* f = (const float*) src;
     for (size_t k = 0; k < n; k++) uold[k] = f[k];
   }
   return 0;
@@ -208,6 +349,10 @@ extern "C" int hydro_amd_server_result(hydro_amd_server* s, int rid, double* uol
 // Drops a kept result (frees host memory).
 extern "C" int hydro_amd_server_release(hydro_amd_server* s, int rid) {
   if (!s || rid < 0 || rid >= (int) s->req.size()) return hydro_amd_set_error("no request %d", rid);
-  std::vector<unsigned char>().swap(s->req[rid].result);
+  Request& r = s->req[rid];
+  if (r.arena_off != ~(size_t) 0) s->arena.release(r.arena_off, r.result_bytes);
+  r.arena_off = ~(size_t) 0;
+  r.result.reset();
+  r.result_bytes = 0;
   return 0;
 }

@@ -169,6 +169,10 @@ int hydro_amd_server_counts(hydro_amd_server *s, int *queued, int *running, int 
 int hydro_amd_server_info(hydro_amd_server *s, int req, double *times, double *t_sim, int *nstep);
 int hydro_amd_server_result(hydro_amd_server *s, int req, double *uold);
 int hydro_amd_server_release(hydro_amd_server *s, int req);
+/* Pin `bytes` of host memory for kept results (once per server): a result
+ * that fits is copied by DMA straight into it while the next chunk runs;
+ * others are copied through staging buffers.  release() returns the space. */
+int hydro_amd_server_reserve_results(hydro_amd_server *s, long long bytes);
 
 /* Measurement support (no counterpart in the reference): runs a plain copy
  * kernel (k_copy_probe) over `bytes` bytes `reps` times and returns GB/s

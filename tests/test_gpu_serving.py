@@ -42,6 +42,29 @@ def test_results_same_alone_and_in_the_mix(precision):
             assert np.array_equal(s.result(rid), u_ref), f"request {rid}"
 
 
+def test_results_through_pinned_arena():
+    """Results copied by DMA into reserved pinned memory, with the arena too
+    small for all of them (the rest take the staged copy), released and
+    reused: every result equals the run alone."""
+    runs = small_runs(8, 6, hydro_amd.MIXED)
+    with hydro_amd.Server(precision=hydro_amd.MIXED, budget_bytes=10**9, chunk_steps=9,
+                          keep_results=True) as s:
+        room = sum(16 * (r.params.nx + 4) * (r.params.ny + 4) + 256 for r in runs[:3])
+        s.reserve_results(room)
+        ids = [s.submit(r.params, r.bx, r.by, r.e) for r in runs]
+        while s.pump() > 0:
+            pass
+        for rid, run in zip(ids, runs):
+            assert np.array_equal(s.result(rid), alone(run)[0]), f"request {rid}"
+        for rid in ids:
+            s.release(rid)
+        again = [s.submit(r.params, r.bx, r.by, r.e) for r in runs[:3]]
+        while s.pump() > 0:
+            pass
+        for rid, run in zip(again, runs[:3]):
+            assert np.array_equal(s.result(rid), alone(run)[0]), f"request {rid} (reused)"
+
+
 def test_budget_and_oversized_request():
     runs = small_runs(6, 4, hydro_amd.MIXED, lo=64, hi=64)
     with hydro_amd.Batch(hydro_amd.MIXED) as b:
