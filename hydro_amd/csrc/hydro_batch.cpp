@@ -23,7 +23,7 @@
 
 using namespace hydro;
 
-int hydro_amd_set_error(const char* fmt, ...);  // hydro_capi.cpp
+extern "C" int hydro_amd_set_error(const char* fmt, ...);  // hydro_capi.cpp
 
 #define BCHK(call)                                                                         \
   do {                                                                                     \

@@ -40,8 +40,9 @@ static int fail(const char* fmt, ...) {
 
 extern "C" const char* hydro_amd_last_error(void) { return g_err.c_str(); }
 
-// For the other translation units of the library (hydro_batch.cpp, ...).
-int hydro_amd_set_error(const char* fmt, ...) {
+// For the other translation units of the library (hydro_batch.cpp, the C99
+// host files, ...); C linkage so the C files can call it.
+extern "C" int hydro_amd_set_error(const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);

@@ -23,7 +23,7 @@
 
 #include "../../include/hydro_amd.h"
 
-int hydro_amd_set_error(const char* fmt, ...);  // hydro_capi.cpp
+extern "C" int hydro_amd_set_error(const char* fmt, ...);  // hydro_capi.cpp
 int hydro_amd_batch_download_raw(hydro_amd_batch* b, int id, void* dst, int* elem_size,
                                  hipStream_t st);  // hydro_batch.cpp
 int hydro_amd_batch_raw_view(hydro_amd_batch* b, int id, const void** src, long* pitch,

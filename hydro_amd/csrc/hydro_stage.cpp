@@ -12,7 +12,7 @@
 
 #include "hydro_kernels.hpp"
 
-int hydro_amd_set_error(const char* fmt, ...);  // hydro_capi.cpp
+extern "C" int hydro_amd_set_error(const char* fmt, ...);  // hydro_capi.cpp
 
 namespace hydro {
 namespace {

@@ -2,6 +2,7 @@
  * hydro -- the HydroC99 driver over libhydro_amd.
  *
  *   hydro -i deck.nml [-v] [-dtlog FILE] [-dump FILE] [-native [-chunk K]]
+ *         [-restart FILE] [-checkpoint FILE]
  *
  * Default mode calls the reference call surface step by step exactly as
  * HydroC99_2DMpi/Src/main.c:244-364 does (dt on even steps, halved at step 0,
@@ -12,6 +13,18 @@
  *
  * -dtlog writes "step t dt" per step with %.17g; -dump writes the final state
  * as an FP64 binary dump (hydro_amd_dump).
+ *
+ * Checkpoint / restart and the time guard follow the C++ variant
+ * (HydroCplusMPI/TimeLimit.cpp:22-55, Domain.cpp:103-122,
+ * Domain_Compute.cpp:531-569): the allotted time comes from HYDROC_MAXTIME
+ * (else BRIDGE_MPRUN_MAXTIME, BRIDGE_MSUB_MAXTIME), less the time already
+ * spent since HYDROC_START_TIME (epoch seconds) if set; the guard is 900 s,
+ * 600 s below an hour, 300 s below 30 min, 20 s below a minute; a run with
+ * at most 1 s allotted aborts.  After each step (each chunk with -native) the
+ * loop stops once the remaining time is below the guard.  Unlike the C++
+ * variant, no limit applies when none of these variables is set (the C99
+ * driver has none).  -checkpoint writes the state, t, nstep and dt at the end
+ * of the loop, however it ended; -restart continues from such a file.
  */
 #include <float.h>
 #include <math.h>
@@ -28,12 +41,24 @@ static double now(void) {
   return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
+/* Seconds allotted to this run (< 0: no limit). */
+static double allotted_time(void) {
+  const char *p = getenv("HYDROC_MAXTIME");
+  if (!p) p = getenv("BRIDGE_MPRUN_MAXTIME");
+  if (!p) p = getenv("BRIDGE_MSUB_MAXTIME");
+  if (!p) return -1.0;
+  double t = strtod(p, NULL);
+  const char *s = getenv("HYDROC_START_TIME");
+  if (s) t -= (double) ((long) time(NULL) - strtol(s, NULL, 10));
+  return t;
+}
+
 int main(int argc, char **argv) {
   hydroparam_t H;
   hydrovar_t Hv;
   hydrowork_t Hw_godunov, Hw_deltat;
   hydrovarwork_t Hvw_godunov, Hvw_deltat;
-  const char *dtlog_path = NULL, *dump_path = NULL;
+  const char *dtlog_path = NULL, *dump_path = NULL, *restart_path = NULL, *ckpt_path = NULL;
   int native = 0, chunk = 100;
   char **pass = (char **) calloc((size_t) argc + 1, sizeof(char *));
   int npass = 0;
@@ -43,13 +68,34 @@ int main(int argc, char **argv) {
     else if (strcmp(argv[n], "-dump") == 0 && n + 1 < argc) dump_path = argv[++n];
     else if (strcmp(argv[n], "-native") == 0) native = 1;
     else if (strcmp(argv[n], "-chunk") == 0 && n + 1 < argc) chunk = atoi(argv[++n]);
+    else if (strcmp(argv[n], "-restart") == 0 && n + 1 < argc) restart_path = argv[++n];
+    else if (strcmp(argv[n], "-checkpoint") == 0 && n + 1 < argc) ckpt_path = argv[++n];
     else pass[npass++] = argv[n];
   }
   if (chunk < 1) chunk = 1;
   memset(functim, 0, sizeof(functim));
   process_args(npass, pass, &H);
   free(pass);
+  const double t_begin = now();
+  const double allotted = allotted_time();
+  double guard = 0;
+  if (allotted >= 0) {
+    if (allotted <= 1) {
+      fprintf(stderr, "Hydro: allocated time too short %g s\n", allotted);
+      abort();
+    }
+    guard = allotted < 60 ? 20 : allotted < 1800 ? 300 : allotted < 3600 ? 600 : 900;
+    fprintf(stdout, "Hydro: allocated time %g s, time guard %g s\n", allotted, guard);
+  }
   hydro_init(&H, &Hv);
+  double dt = 0;
+  if (restart_path) {
+    if (hydro_amd_restore(restart_path, &H, &Hv, &dt) != 0) {
+      fprintf(stderr, "hydro: %s\n", hydro_amd_last_error());
+      return 1;
+    }
+    fprintf(stdout, "Hydro restarts at step %d, t = %.17g\n", H.nstep, H.t);
+  }
   fprintf(stdout, "Hydro starts in double precision.\n");
   fprintf(stdout, "Hydro: libhydro_amd on gfx950 (%s loop)\n", native ? "device" : "host");
 
@@ -59,16 +105,21 @@ int main(int argc, char **argv) {
   if (H.dtoutput > 0) {
     time_output = 1;
     next_output_time += H.dtoutput;
+    while (next_output_time <= H.t) next_output_time += H.dtoutput;  /* after a restart */
   }
   if (H.dtoutput > 0 || H.noutput > 0) vtkfile(++nvtk, H, &Hv);
   fprintf(stdout, "Hydro starts main loop.\n");
   allocate_work_space(H.nxyt, H, &Hw_godunov, &Hvw_godunov);
   compute_deltat_init_mem(H, &Hw_deltat, &Hvw_deltat);
 
-  double dt = 0, mcsavg = 0, mcsmin = FLT_MAX, mcsmax = 0, mcssig = 0;
+  double mcsavg = 0, mcsmin = FLT_MAX, mcsmax = 0, mcssig = 0;
   long nmcs = 0;
   const double t_start = now();
   hydro_amd_grid *g = native ? hydro_amd_surface_grid(&Hv) : NULL;
+  if (g && hydro_amd_grid_set_state(g, H.t, H.nstep, dt) != 0) {
+    fprintf(stderr, "hydro: %s\n", hydro_amd_last_error());
+    return 1;
+  }
   while ((H.t < H.tend) && (H.nstep < H.nstepmax)) {
     double t0 = now();
     int steps = 1;
@@ -132,9 +183,23 @@ int main(int argc, char **argv) {
     }
     if (native) free(chunk_dt);
     fflush(stdout);
+    if (allotted >= 0 && (H.t < H.tend) && (H.nstep < H.nstepmax)) {
+      const double remain = allotted - (now() - t_begin);
+      if (remain < guard) {
+        fprintf(stderr, " Hydro stops by time limit %g < %g\n", remain, guard);
+        break;
+      }
+    }
   }
   const double elaps = now() - t_start;
   if (dtlog) fclose(dtlog);
+  if (ckpt_path) {
+    if (hydro_amd_checkpoint(ckpt_path, H, &Hv, dt) != 0) {
+      fprintf(stderr, "hydro: %s\n", hydro_amd_last_error());
+      return 1;
+    }
+    fprintf(stdout, "Hydro: checkpoint at step %d written to %s\n", H.nstep, ckpt_path);
+  }
   if (dump_path && hydro_amd_dump(dump_path, H, &Hv) != 0)
     fprintf(stderr, "hydro: cannot write %s\n", dump_path);
   deallocate_work_space(H.nxyt, H, &Hw_godunov, &Hvw_godunov);

@@ -25,6 +25,8 @@
 
 #include "../../include/hydro_amd.h"
 
+int hydro_amd_set_error(const char *fmt, ...); /* hydro_capi.cpp */
+
 static char six_bits(unsigned b) {
   if (b < 26) return (char) ('A' + b);
   if (b < 52) return (char) ('a' + b - 26);
@@ -174,18 +176,67 @@ void vtkfile(int step, const hydroparam_t H, hydrovar_t *Hv) {
   }
 }
 
-int hydro_amd_dump(const char *path, const hydroparam_t H, hydrovar_t *Hv) {
-  (void) hydro_amd_sync_host(Hv);
+static int write_state(const char *path, const hydroparam_t H, hydrovar_t *Hv, double dt) {
+  if (hydro_amd_sync_host(Hv) != 0) return -1;
   FILE *f = fopen(path, "wb");
   if (!f) return -1;
   const int32_t hdr[4] = {H.nx, H.ny, 4, H.nstep};
-  const double td[2] = {H.t, 0.0};
-  fwrite("HYDROF64", 1, 8, f);
-  fwrite(hdr, sizeof(int32_t), 4, f);
-  fwrite(td, sizeof(double), 2, f);
+  const double td[2] = {H.t, dt};
+  int ok = fwrite("HYDROF64", 1, 8, f) == 8 && fwrite(hdr, sizeof(int32_t), 4, f) == 4 &&
+           fwrite(td, sizeof(double), 2, f) == 2;
   const size_t nxt = (size_t) H.nxt, nyt = (size_t) H.nyt;
-  for (int v = 0; v < 4; v++)
-    for (size_t j = 2; j < nyt - 2; j++)
-      fwrite(Hv->uold + 2 + nxt * (j + nyt * v), sizeof(double), (size_t) H.nx, f);
-  return fclose(f) == 0 ? 0 : -1;
+  for (int v = 0; v < 4 && ok; v++)
+    for (size_t j = 2; j < nyt - 2 && ok; j++)
+      ok = fwrite(Hv->uold + 2 + nxt * (j + nyt * v), sizeof(double), (size_t) H.nx, f) ==
+           (size_t) H.nx;
+  return (fclose(f) == 0 && ok) ? 0 : -1;
+}
+
+int hydro_amd_dump(const char *path, const hydroparam_t H, hydrovar_t *Hv) {
+  return write_state(path, H, Hv, 0.0);
+}
+
+int hydro_amd_checkpoint(const char *path, const hydroparam_t H, hydrovar_t *Hv, double dt) {
+  char prev[4096];
+  if (snprintf(prev, sizeof(prev), "%s.prev", path) < (int) sizeof(prev)) {
+    struct stat st;
+    if (stat(path, &st) == 0) (void) rename(path, prev);
+  }
+  if (write_state(path, H, Hv, dt) != 0) {
+    hydro_amd_set_error("cannot write checkpoint %s", path);
+    return -1;
+  }
+  return 0;
+}
+
+int hydro_amd_restore(const char *path, hydroparam_t *H, hydrovar_t *Hv, double *dt) {
+  FILE *f = fopen(path, "rb");
+  if (!f) {
+    hydro_amd_set_error("cannot open checkpoint %s", path);
+    return -1;
+  }
+  char magic[8];
+  int32_t hdr[4];
+  double td[2];
+  int ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "HYDROF64", 8) == 0 &&
+           fread(hdr, sizeof(int32_t), 4, f) == 4 && fread(td, sizeof(double), 2, f) == 2;
+  if (!ok || hdr[0] != H->nx || hdr[1] != H->ny || hdr[2] != 4 || hdr[3] < 0) {
+    fclose(f);
+    hydro_amd_set_error("checkpoint %s does not match this %dx%d deck", path, H->nx, H->ny);
+    return -1;
+  }
+  const size_t nxt = (size_t) H->nxt, nyt = (size_t) H->nyt;
+  for (int v = 0; v < 4 && ok; v++)
+    for (size_t j = 2; j < nyt - 2 && ok; j++)
+      ok = fread(Hv->uold + 2 + nxt * (j + nyt * v), sizeof(double), (size_t) H->nx, f) ==
+           (size_t) H->nx;
+  fclose(f);
+  if (!ok) {
+    hydro_amd_set_error("checkpoint %s is truncated", path);
+    return -1;
+  }
+  H->nstep = hdr[3];
+  H->t = td[0];
+  if (dt) *dt = td[1];
+  return hydro_amd_sync_device(Hv);
 }

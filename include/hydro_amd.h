@@ -191,6 +191,19 @@ hydro_amd_grid *hydro_amd_surface_grid(hydrovar_t *Hv);
  * double[nvar][ny][nx].  Refreshes the host mirror first. */
 int hydro_amd_dump(const char *path, const hydroparam_t H, hydrovar_t *Hv);
 
+/* Checkpoint / restart.  HydroC99_2DMpi has none; this keeps what the C++
+ * variant's Continue.dump keeps for one domain (HydroCplusMPI/
+ * Domain_Prot.cpp:98-181: the state, t, dt, the step count) in the dump's
+ * layout, with dt in the second double.  An existing file is first renamed
+ * to path.prev (Domain_Prot.cpp:268-282).  Ghost cells are not stored: every
+ * sweep refills the ghosts it reads. */
+int hydro_amd_checkpoint(const char *path, const hydroparam_t H, hydrovar_t *Hv, double dt);
+/* Loads a checkpoint into a state built by hydro_init from the same deck
+ * (nx, ny must match): the interior of Hv->uold, H->t, H->nstep and *dt (the
+ * dt an odd step reuses); refreshes the GPU copy.  0, or -1 with
+ * hydro_amd_last_error() set. */
+int hydro_amd_restore(const char *path, hydroparam_t *H, hydrovar_t *Hv, double *dt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,3 +63,38 @@ def test_reference_driver_on_gpu_library(tmp_path, testcase):
         assert vtk_files(dirs[k]) == files, k
         for f in files:
             assert filecmp.cmp(dirs["ref"] / f, dirs[k] / f, shallow=False), (k, f)
+
+
+def run_env(exe, deck, cwd, env, *extra):
+    out = subprocess.run([exe, "-i", deck, *extra], cwd=cwd, capture_output=True, text=True,
+                         timeout=300, env=dict(os.environ, OMP_NUM_THREADS="4", **env))
+    assert out.returncode == 0, out.stderr[-2000:]
+    return re.findall(r"--> step=\s*(\d+),\s*(\S+),\s*(\S+)", out.stdout), out
+
+
+@pytest.mark.parametrize("mode", [(), ("-native", "-chunk", "3")])
+def test_time_guard_checkpoint_restart(tmp_path, mode):
+    """The time guard stops a run (forced from the host: HYDROC_START_TIME
+    leaves 15 s of a 1000 s allotment, below the 20 s guard), the checkpoint
+    is written, and the restarted run ends bit-identical to an uninterrupted
+    one, with the same step lines from the restart on."""
+    import time
+    import numpy as np
+    p = oracle.Params(nx=96, ny=64, dx=1 / 96.0, courant_factor=0.8, tend=1e30, nstepmax=25,
+                      testcase=1)
+    deck = oracle.write_deck(p, str(tmp_path / "deck.nml"))
+    full, _ = run_env(HYDRO, deck, tmp_path, {}, *mode, "-dump", "full.bin")
+    assert len(full) == 25
+    env = {"HYDROC_MAXTIME": "1000", "HYDROC_START_TIME": str(int(time.time()) - 985)}
+    first, out = run_env(HYDRO, deck, tmp_path, env, *mode, "-checkpoint", "ck.bin")
+    assert "stops by time limit" in out.stderr
+    k = len(first)
+    assert 1 <= k < 25 and first == full[:k]
+    # a second checkpoint keeps the first as .prev
+    second, _ = run_env(HYDRO, deck, tmp_path, {}, *mode, "-restart", "ck.bin",
+                        "-checkpoint", "ck.bin", "-dump", "rest.bin")
+    assert os.path.exists(tmp_path / "ck.bin.prev")
+    assert second == full[k:]
+    a = np.fromfile(tmp_path / "full.bin", np.uint8)
+    b = np.fromfile(tmp_path / "rest.bin", np.uint8)
+    assert np.array_equal(a, b)
