@@ -80,7 +80,7 @@ def gpu_bench(nx, ny, steps, warmup, precision, autotune=True, testcase=0):
 
 
 def batched_bench(n=512, steps=100):
-    """Config 3: n seeded blast instances (sides log-uniform in [64, 1024]),
+    """Batched workload: n seeded blast instances (sides log-uniform in [64, 1024]),
     `steps` steps each: the plain per-grid FP64 loop (one Grid per instance,
     created, run and freed in turn) against one batch (FP64 and mixed)."""
     import hydro_amd
@@ -141,7 +141,7 @@ def _serve(runs, chunk):
 
 
 def serving_bench(n=32, rates=(20.0, 1000.0), chunk=20):
-    """Config 4: n runs (sides log-uniform in [128, 4096], t_end 0.1 or 500
+    """Serving workload: n runs (sides log-uniform in [128, 4096], t_end 0.1 or 500
     steps) arriving as a Poisson process; mixed precision served by the
     continuous-batching loop against the plain per-grid FP64 loop (FIFO, one
     run at a time, each run's service time measured) on the same requests.
@@ -349,7 +349,7 @@ def main():
                     help="skip the two rocprofv3 counter passes behind roofline.traffic")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); the "
+        print(f"bench.py: runs on one GPU only (asked for --gpus {a.gpus}); the "
               "multi-GPU row-band decomposition is a design, not built", file=sys.stderr)
         sys.exit(2)
     ny = a.ny or a.nx

@@ -1,7 +1,7 @@
 """The oracle: a CPU restatement of HydroC99's split Godunov/MUSCL step.
 
 TEST INFRASTRUCTURE ONLY.  ``tests/``, ``__graft_entry__.smoke()`` and the
-CPU-baseline leg of ``bench.py`` import this package as the checker.  The
+CPU-baseline leg of ``bench.py`` import this package, as the reference check.  The
 product (``hydro_amd``) never imports it, and nothing measured runs through it.
 
 Two implementations sit behind the same Python surface:

@@ -2,7 +2,7 @@
  * hydro_oracle.c -- CPU restatement of HydroC99's split Godunov/MUSCL step.
  *
  * TEST INFRASTRUCTURE ONLY (the oracle): used by tests/, the smoke check and
- * bench.py's CPU-baseline leg as the checker; never linked into the product.
+ * bench.py's CPU-baseline leg as the reference check; never linked into the product.
  *
  * Every function restates one piece of HydroC/HydroC99_2DMpi/Src and cites the
  * lines it follows.  The operation order of each expression is the reference's

@@ -2,7 +2,7 @@
  * hydro_oracle.h -- CPU restatement of HydroC99's split Godunov/MUSCL step.
  *
  * TEST INFRASTRUCTURE ONLY.  This is the oracle: tests, the smoke check and the
- * CPU-baseline leg of bench.py use it as the checker.  The product path (the
+ * CPU-baseline leg of bench.py use it as the reference check.  The product path (the
  * HIP library) never links or calls it.
  *
  * Layout follows the reference so arrays compare element for element:

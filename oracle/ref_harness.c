@@ -2,7 +2,7 @@
  * Test-only harness around the reference HydroC99_2DMpi sources.
  *
  * Built by oracle/Makefile into oracle/_ref/libhydroref.so together with the
- * reference's own Src/*.c (read in place, never copied).  It is the checker:
+ * reference's own Src/*.c (read in place, never copied).  Test infrastructure:
  * tests and the golden-vector script call it through ctypes to (1) run the
  * reference's stage functions on chosen slabs and (2) drive the reference's
  * time loop with full-precision access to uold and the dt sequence, which the

@@ -177,10 +177,10 @@ def main():
     oracle.build(ref=True)
     ref = oracle.Ref()
     stage_vectors(ref)
-    # config 0: 250x125 corner blast, dx = 1/250, CFL 0.8, 100 steps
-    c0 = Params(nx=250, ny=125, dx=1.0 / 250, courant_factor=0.8, tend=1e30, nstepmax=100,
+    # 250x125 corner blast, dx = 1/250, CFL 0.8, 100 steps
+    corner = Params(nx=250, ny=125, dx=1.0 / 250, courant_factor=0.8, tend=1e30, nstepmax=100,
                 testcase=1, nxystep=125)
-    trajectory(ref, "c0", c0, 100)
+    trajectory(ref, "corner250x125", corner, 100)
     # the reference's own deck Input/input_250x125_corner.nml (dx 0.05), 100 steps
     deck = Params(nx=250, ny=125, dx=0.05, courant_factor=0.8, tend=200.0, nstepmax=100,
                   testcase=1, nxystep=125)
@@ -204,7 +204,7 @@ def main():
         trajectory(ref, name, Params(nx=24, ny=17, dx=1 / 24.0, courant_factor=0.8, tend=1e30,
                                      testcase=0, **kw), 40)
     # batched-style instances: random gamma and blast cell / energy (BASELINE
-    # configs 3-4 physics), the reference run with gamma set on its struct
+    # the batched and serving workloads' physics), the reference run with gamma set on its struct
     rng = np.random.default_rng(20261202)
     for k in range(3):
         n = int(rng.integers(20, 48))

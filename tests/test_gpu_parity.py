@@ -11,7 +11,7 @@ import pytest
 
 import hydro_amd
 import oracle
-from helpers import C0, rel_l1, to_oracle
+from helpers import CORNER_250x125, rel_l1, to_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -48,8 +48,8 @@ def surface_loop(g, steps):
 
 
 def test_c0_host_loop_100_steps(port):
-    """Config 0 (250x125 corner blast), 100 steps, host-driven loop."""
-    p = hydro_amd.Params(**C0)
+    """The 250x125 corner blast, 100 steps, host-driven loop."""
+    p = hydro_amd.Params(**CORNER_250x125)
     with hydro_amd.Grid(p) as g:
         g.init(1)
         dts = surface_loop(g, 100)
@@ -63,7 +63,7 @@ def test_c0_host_loop_100_steps(port):
 
 def test_c0_device_loop_100_steps(port):
     """The same run as a device-side loop (hipGraph replay); t and nstep too."""
-    p = hydro_amd.Params(**C0)
+    p = hydro_amd.Params(**CORNER_250x125)
     with hydro_amd.Grid(p) as g:
         g.init(1)
         dts = g.run(100)
@@ -80,12 +80,12 @@ def test_c0_device_loop_100_steps(port):
 
 def test_device_loop_chunks_and_tend(port):
     """Runs split into odd-sized chunks and stopped by tend give the same steps."""
-    p = hydro_amd.Params(**{**C0, "nx": 61, "ny": 67, "dx": 1 / 61.0})
+    p = hydro_amd.Params(**{**CORNER_250x125, "nx": 61, "ny": 67, "dx": 1 / 61.0})
     po = to_oracle(p, testcase=0)
     ur = port.init(po)
     rdt, (rt, _, _) = port.run(po, ur, 40)
     tend = float(np.cumsum(rdt)[24])  # stops after step 25 (t >= tend)
-    p2 = hydro_amd.Params(**{**C0, "nx": 61, "ny": 67, "dx": 1 / 61.0, "tend": tend})
+    p2 = hydro_amd.Params(**{**CORNER_250x125, "nx": 61, "ny": 67, "dx": 1 / 61.0, "tend": tend})
     po2 = to_oracle(p2, testcase=0)
     ur2 = port.init(po2)
     rdt2, (rt2, rn2, _) = port.run(po2, ur2, 1000)
@@ -135,7 +135,7 @@ VARIANTS = {
 
 @pytest.mark.parametrize("name", sorted(VARIANTS))
 def test_scheme_variants(port, name):
-    p = hydro_amd.Params(**{**C0, "nx": 97, "ny": 70, "dx": 1 / 97.0, **VARIANTS[name]})
+    p = hydro_amd.Params(**{**CORNER_250x125, "nx": 97, "ny": 70, "dx": 1 / 97.0, **VARIANTS[name]})
     po = to_oracle(p, testcase=0)
     ur = port.init(po)
     rdt, _ = port.run(po, ur, 30)
@@ -149,7 +149,7 @@ def test_scheme_variants(port, name):
 
 def test_surface_calls_on_uploaded_state(port):
     """make_boundary / compute_deltat / godunov on an arbitrary uploaded state."""
-    p = hydro_amd.Params(**{**C0, "nx": 77, "ny": 45, "dx": 1 / 77.0,
+    p = hydro_amd.Params(**{**CORNER_250x125, "nx": 77, "ny": 45, "dx": 1 / 77.0,
                             "boundary_left": 1, "boundary_right": 2,
                             "boundary_down": 3, "boundary_up": 1})
     po = to_oracle(p, testcase=1)
@@ -177,7 +177,7 @@ def test_surface_calls_on_uploaded_state(port):
                                                      (1000, 3, 2, 64)])
 def test_tiling_independence(port, yseg, ywaves, xwaves, xiter):
     """Results do not depend on the sweep launch shapes."""
-    p = hydro_amd.Params(**{**C0, "nx": 130, "ny": 150, "dx": 1 / 130.0})
+    p = hydro_amd.Params(**{**CORNER_250x125, "nx": 130, "ny": 150, "dx": 1 / 130.0})
     po = to_oracle(p, testcase=0)
     ur = port.init(po)
     rdt, _ = port.run(po, ur, 10)
@@ -192,7 +192,7 @@ def test_tiling_independence(port, yseg, ywaves, xwaves, xiter):
 
 def test_autotune_keeps_state_and_results(port):
     """The autotuner only writes scratch: the state and the run are unchanged."""
-    p = hydro_amd.Params(**{**C0, "nx": 300, "ny": 200, "dx": 1 / 300.0})
+    p = hydro_amd.Params(**{**CORNER_250x125, "nx": 300, "ny": 200, "dx": 1 / 300.0})
     po = to_oracle(p, testcase=1)
     ur = port.init(po)
     rdt, _ = port.run(po, ur, 8)
@@ -210,7 +210,7 @@ def test_autotune_keeps_state_and_results(port):
 
 def test_mid_size_against_oracle(port):
     """1024x768 centred blast, 20 steps, bitwise."""
-    p = hydro_amd.Params(**{**C0, "nx": 1024, "ny": 768, "dx": 1 / 1024.0})
+    p = hydro_amd.Params(**{**CORNER_250x125, "nx": 1024, "ny": 768, "dx": 1 / 1024.0})
     po = to_oracle(p, testcase=0)
     ur = port.init(po)
     rdt, _ = port.run(po, ur, 20)

@@ -1,6 +1,6 @@
 """The oracle pinned: golden vectors, the live reference, and invariants (CPU).
 
-Three independent anchors for oracle/hydro_oracle.c (the checker every GPU
+Three independent anchors for oracle/hydro_oracle.c (the oracle every GPU
 test compares against):
  1. the golden vectors written by tests/golden/make_golden.py from the
     reference's own compiled sources -- every stage, and whole trajectories
