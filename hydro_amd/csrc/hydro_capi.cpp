@@ -79,7 +79,7 @@ struct hydro_amd_grid {
   long state_ver = 1;
   long cour_ver = 0;     // state version whose CFL maxima the slots hold (0: none)
   bool cour_dirty = false;
-  SweepTune tune = {64, 4, 4, 8};
+  SweepTune tune = {64, 1, 2, 8};  // the autotuner's usual pick at 8192^2
   // host mirror of the run control, valid between calls
   double t = 0, dt = 0;
   int nstep = 0;
