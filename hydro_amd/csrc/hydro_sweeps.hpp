@@ -222,10 +222,14 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
   S cD = pd.c;
   Q4<S> qxmC = qxm_c;
   Cell<S> uC = uc, uD = ud;
-  // One row of the march: load row j, update row j-2, shift the window.
+  // the next row's loads are issued one row ahead of their use
+  Cell<S> uN = load_y(in, (long) (j0 + 2) * P + i);
+  // One row of the march: take row j, prefetch row j+1, update row j-2,
+  // shift the window.
   auto row = [&](int j) {
-    const long oj = (long) j * P + i;
-    const Cell<S> uE = load_y(in, oj);
+    const Cell<S> uE = uN;
+    const int jn = j + 1 <= j1 + 1 ? j + 1 : j1 + 1;  // last row: a harmless reload
+    uN = load_y(in, (long) jn * P + i);
     const Prim<S> pE = prim_cell<S>(uE.u0, uE.u1, uE.u2, uE.u3, ph);
     Q4<S> qxmD, qxpD;
     trace_cell<S>(qD, o1 ? z : slope4<S, ST1>(qC, qD, pE.q, st), cD, tk, qxmD, qxpD);
