@@ -1,0 +1,60 @@
+"""The benchmark grids at their full size, on the GPU, against the oracle.
+
+The 8192x8192 centred blast and the 23000x23000 corner blast (dx = 1/n, CFL
+0.8, reflective walls) are compared bit for bit with the oracle's OpenMP
+restatement over the first steps (both sweep orders, the halved first dt and
+the first dt reuse), and the 23000x23000 state is checked for conservation
+of mass and total energy (reflective walls: conserved to round-off).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import hydro_amd
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(n, testcase, steps):
+    p = hydro_amd.Params(nx=n, ny=n, dx=1.0 / n, courant_factor=0.8, tend=1e30)
+    with hydro_amd.Grid(p) as g:
+        g.init(testcase)
+        dts = g.run(steps)
+        u = g.download()
+    po = oracle.Params(nx=n, ny=n, dx=1.0 / n, courant_factor=0.8, tend=1e30, testcase=testcase)
+    port = oracle.Port()
+    ur = port.init(po)
+    rdts, _ = port.run(po, ur, steps)
+    return u, dts, ur, rdts
+
+
+def test_8192_centred_blast_bitwise():
+    u, dts, ur, rdts = run_both(8192, 0, 6)
+    assert np.array_equal(dts, rdts), (dts, rdts)
+    bad = np.count_nonzero(u != ur)
+    assert bad == 0, f"{bad} values differ"
+
+
+@pytest.mark.skipif(os.environ.get("HYDRO_SKIP_23000") == "1", reason="asked to skip")
+def test_23000_corner_blast_bitwise_and_conserved():
+    n = 23000
+    u, dts, ur, rdts = run_both(n, 1, 2)
+    assert np.array_equal(dts, rdts), (dts, rdts)
+    for v in range(4):
+        bad = np.count_nonzero(u[v] != ur[v])
+        assert bad == 0, f"variable {v}: {bad} values differ"
+    del ur
+    # conservation: interior sums of rho and E against the initial state
+    # (rho = 1 everywhere, E = 1e-5 plus the blast energy (1/dx)/dx in one cell)
+    # (row sums, then an exactly rounded sum of the rows: a running float64
+    # total over 23000 rows would itself drift by ~1e-12 of the total)
+    dx = 1.0 / n
+    mass0 = float(n) * n
+    energy0 = math.fsum([1e-5 * (float(n) * n - 1), (1.0 / dx) / dx])
+    mass = math.fsum(np.sum(u[0, 2:-2, 2:-2], axis=1))
+    energy = math.fsum(np.sum(u[3, 2:-2, 2:-2], axis=1))
+    assert abs(mass - mass0) / mass0 < 1e-13, (mass, mass0)
+    assert abs(energy - energy0) / energy0 < 1e-13, (energy, energy0)
