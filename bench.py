@@ -13,9 +13,16 @@ Prints ONE JSON line.  `value` is interior cell-updates per second (the
 reference's MC/s unit x 1e6).  `roofline` is for the dominant sweep kernel:
 algorithmic bytes (64 B per cell per sweep in FP64: read 4 doubles, write 4)
 over its average duration, timed with HIP events recorded on the grid's own
-stream around every launch in the timed region.  `cpu_baseline` is the
-reference's OpenMP CPU path (HydroC99_2Domp, built by oracle/Makefile into
-oracle/_ref/hydro_omp) on a bounded sample of the same workload.
+stream around every launch in the timed region; its `traffic` is the HBM bytes
+the kernel really moves (two rocprofv3 --pmc passes, calibrated on a copy
+kernel).  `valu` is that kernel's VALU work from a third counter pass: VALU
+instructions per cell, FP64 FLOP/s against the 78.6 TFLOP/s vector peak, and
+the share of SIMD cycles issuing VALU (the sweeps are VALU-bound).
+`batched` and `serving` time the many-grid workloads (512 batched instances;
+Poisson arrivals at a nominal and a saturating rate, mixed precision, against
+the plain per-grid FP64 loop).  `cpu_baseline` is the reference's OpenMP CPU
+path (HydroC99_2Domp, built by oracle/Makefile into oracle/_ref/hydro_omp) on
+a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -275,16 +282,17 @@ def pmc_workload(nx, precision, steps, tune=None):
 COPY_PROBE_BYTES = 1 << 30  # each of read and written by pmc_workload's copy kernel
 
 
-def _pmc_kib(counter, args):
-    """Mean of one rocprofv3 counter (KiB per dispatch) per kernel over a run
-    of this script with `args`, or None if the profiler is absent or fails."""
+def _pmc(counters, args):
+    """{counter: {kernel: mean value per dispatch}} from one rocprofv3 --pmc
+    run of this script with `args` (kernels: copy, x_sweep, y_sweep), or None
+    if the profiler is absent or fails."""
     import csv
     import shutil
     exe = shutil.which("rocprofv3")
     if not exe:
         return None
     with tempfile.TemporaryDirectory(dir="/tmp") as d:
-        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+        cmd = [exe, "--pmc", *counters, "--output-format", "csv", "-d", d, "-o", "run", "--",
                sys.executable, os.path.abspath(__file__), "--pmc-workload"] + args
         try:
             subprocess.run(cmd, cwd=d, env=dict(os.environ, TMPDIR="/tmp"), timeout=150,
@@ -297,14 +305,18 @@ def _pmc_kib(counter, args):
         acc = {}
         with open(path) as f:
             for r in csv.DictReader(f):
-                if r.get("Counter_Name") != counter:
-                    continue
                 name = r["Kernel_Name"]
                 key = ("copy" if "k_copy_probe" in name else "x_sweep" if "k_sweep_x" in name
                        else "y_sweep" if "k_sweep_y" in name else None)
-                if key:
-                    acc.setdefault(key, []).append(float(r["Counter_Value"]))
-        return {k: sum(v) / len(v) for k, v in acc.items()}
+                if key and r.get("Counter_Name") in counters:
+                    acc.setdefault(r["Counter_Name"], {}).setdefault(key, []).append(
+                        float(r["Counter_Value"]))
+        return {c: {k: sum(v) / len(v) for k, v in m.items()} for c, m in acc.items()}
+
+
+def _pmc_kib(counter, args):
+    r = _pmc([counter], args)
+    return r.get(counter) if r else None
 
 
 def measured_traffic(nx, precision, tune):
@@ -323,6 +335,38 @@ def measured_traffic(nx, precision, tune):
         if k in fetch and k in write:
             out[k] = (fetch[k] * 1024 * out["fetch_scale"] + write[k] * 1024 * out["write_scale"])
     return out
+
+
+VALU_COUNTERS = ["SQ_INSTS_VALU", "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_MUL_F64",
+                 "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_TRANS_F64", "GRBM_GUI_ACTIVE"]
+FP64_VALU_PEAK = 78.6e12  # MI355X FP64 vector FLOP/s
+SIMDS = 1024              # 256 CUs x 4 SIMDs
+XCDS = 8                  # GRBM_GUI_ACTIVE is summed over the XCDs
+
+
+def measured_valu(nx, precision, tune, kern, avg_s):
+    """VALU work of the sweep kernel `kern` from one rocprofv3 --pmc pass over
+    pmc_workload: VALU instructions per cell, FP64 FLOP/s (FMA = 2) over the
+    kernel's measured duration `avg_s`, and the share of SIMD cycles issuing
+    VALU (each wave64 instruction holds a 16-lane SIMD for 4 cycles)."""
+    args = ["--steps", "2", "--nx", str(nx), "--precision", ["fp64", "mixed"][precision]]
+    if tune:
+        args += ["--tune", ",".join(str(tune[k]) for k in ("yseg", "ywaves", "xwaves", "xiter"))]
+    r = _pmc(VALU_COUNTERS, args)
+    if not r or any(kern not in r.get(c, {}) for c in VALU_COUNTERS):
+        return None
+    v = {c: r[c][kern] for c in VALU_COUNTERS}
+    cells = nx * nx
+    flops = 64 * (2 * v["SQ_INSTS_VALU_FMA_F64"] + v["SQ_INSTS_VALU_MUL_F64"]
+                  + v["SQ_INSTS_VALU_ADD_F64"] + v["SQ_INSTS_VALU_TRANS_F64"])
+    cycles = v["GRBM_GUI_ACTIVE"] / XCDS
+    return {"kernel": kern,
+            "valu_instr_per_cell": round(64 * v["SQ_INSTS_VALU"] / cells, 1),
+            "fp64_flops_per_cell": round(flops / cells, 1),
+            "fp64_tflops": round(flops / avg_s / 1e12, 2),
+            "fp64_peak_tflops": FP64_VALU_PEAK / 1e12,
+            "fp64_frac": round(flops / avg_s / FP64_VALU_PEAK, 4),
+            "valu_issue_busy": round(v["SQ_INSTS_VALU"] * 4 / (SIMDS * cycles), 3)}
 
 
 def main():
@@ -346,7 +390,7 @@ def main():
                     help="short fixed run for rocprofv3 counter passes")
     ap.add_argument("--tune", default="", help="yseg,ywaves,xwaves,xiter (pmc workload)")
     ap.add_argument("--no-traffic", action="store_true",
-                    help="skip the two rocprofv3 counter passes behind roofline.traffic")
+                    help="skip the rocprofv3 counter passes behind roofline.traffic and valu")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: runs on one GPU only (asked for --gpus {a.gpus}); the "
@@ -390,6 +434,9 @@ def main():
             line["roofline"]["traffic_note"] = (
                 "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by "
                 f"{tr['fetch_scale']:.3f} / {tr['write_scale']:.3f} (copy-kernel calibration)")
+        va = measured_valu(a.nx, prec, r["tune"], kern, r["kernels"][kern]["avg_ms"] / 1e3)
+        if va:
+            line["valu"] = va
     if not a.no_batched:
         line["batched"] = batched_bench(a.batch_n)
     if not a.no_serving:
