@@ -51,7 +51,7 @@ def gpu_bench(nx, ny, steps, warmup, precision, autotune=True, testcase=0):
     g = hydro_amd.Grid(p)
     g.init(testcase)
     # launch shapes picked on this grid before timing (the state is untouched)
-    tune = g.autotune(reps=2) if autotune else None
+    tune = g.autotune(reps=5) if autotune else None
     if warmup:
         g.run(warmup, log=False)
     g.set_profile(True)

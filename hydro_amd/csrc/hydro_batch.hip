@@ -6,9 +6,11 @@
 // A step is two phases.  In each phase every instance does one sweep, x or y
 // depending on its own step parity (even steps x then y, odd steps y then x,
 // HydroC99_2DMpi/Src/main.c:285-291), so a phase's task table mixes x tasks
-// of some instances with y tasks of others.  Workgroups are persistent: the
-// grid is sized to the GPU, and each workgroup walks the task table with a
-// stride.  A task is one workgroup's worth of one instance:
+// of some instances with y tasks of others (the x and y tasks run as separate
+// launches, and each list as two: instances with slope_type == 1 first, in
+// the specialised variant).  Workgroups are persistent: the grid is sized to
+// the GPU, and the workgroups pull task indices from an atomic counter.  A
+// task is one workgroup's worth of one instance:
 //   x task: wpb * xiter consecutive 60-cell row segments (each wave walks xiter)
 //   y task: 64 columns x (wpb * yseg) rows (each wave marches yseg rows)
 // so the CFL reduction of a task never mixes instances.  Run control (t, dt,

@@ -8,8 +8,9 @@
 //   slope, trace, Riemann solve, flux and update all stay in registers.
 // * y sweep: one lane owns one column and marches down a segment of rows with
 //   a three-cell register window (primitives, one carried trace state, one
-//   carried flux).  A wave covers 64 adjacent columns, so every load/store is
-//   one contiguous row piece.
+//   carried flux), two rows per trip, loading each row one row ahead of its
+//   use.  A wave covers 64 adjacent columns, so every load/store is one
+//   contiguous row piece.
 // * Both read one state buffer and write the other (ping-pong).
 // * CFL maxima of the written state can be folded into the epilogue: reduced
 //   wave -> workgroup -> one 64-bit atomic max per workgroup into one of a
