@@ -203,8 +203,8 @@ def main():
     for name, kw in variants.items():
         trajectory(ref, name, Params(nx=24, ny=17, dx=1 / 24.0, courant_factor=0.8, tend=1e30,
                                      testcase=0, **kw), 40)
-    # batched-style instances: random gamma and blast cell / energy (BASELINE
-    # the batched and serving workloads' physics), the reference run with gamma set on its struct
+    # instances like the batched and serving workloads' (random gamma, blast
+    # cell and energy); the reference runs with gamma set on its struct
     rng = np.random.default_rng(20261202)
     for k in range(3):
         n = int(rng.integers(20, 48))

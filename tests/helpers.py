@@ -6,9 +6,10 @@ import numpy as np
 import hydro_amd
 import oracle
 
-# 250x125 corner blast, CFL 0.8, 100 steps
-CORNER_250x125 = dict(nx=250, ny=125, dx=1.0 / 250, courant_factor=0.8, niter_riemann=10, iorder=2,
-          slope_type=1.0, smallr=1e-10, smallc=1e-10, tend=1e30, nstepmax=100)
+# 250x125 corner blast (E = 1/dx^2 in interior cell (0, 0)), CFL 0.8, 100 steps
+CORNER_250x125 = dict(nx=250, ny=125, dx=1.0 / 250, courant_factor=0.8, niter_riemann=10,
+                      iorder=2, slope_type=1.0, smallr=1e-10, smallc=1e-10, tend=1e30,
+                      nstepmax=100)
 
 
 def to_oracle(p: hydro_amd.Params, testcase: int = 0) -> oracle.Params:
