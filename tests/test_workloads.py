@@ -31,3 +31,19 @@ def test_serving_requests_poisson():
     sides = np.array([r.params.nx for r in runs])
     assert sides.min() >= 128 and sides.max() <= 4096
     assert all(r.params.tend == 0.1 and r.params.nstepmax == 500 for r in runs)
+
+
+def test_generators_reproduce_committed_lists():
+    """The batched and serving instance lists equal the committed fixture
+    (tests/golden/workloads.json) bit for bit."""
+    import json
+    import os
+    fix = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "workloads.json")))
+
+    def rows(runs, arrivals=False):
+        return [[r.params.nx, repr(r.params.gamma), r.bx, r.by, repr(r.e)]
+                + ([repr(r.arrival)] if arrivals else []) for r in runs]
+
+    assert rows(W.batch_instances(512)) == fix["batch_instances"]["rows"]
+    for rate, f in fix["serving_requests"].items():
+        assert rows(W.serving_requests(32, float(rate)), arrivals=True) == f["rows"]
