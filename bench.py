@@ -208,7 +208,11 @@ def serving_bench(n=32, rates=(20.0, 1000.0), chunk=20):
     res["mixed_error_vs_fp64"] = {"density_l1_max": float(errs[:, 0, 0].max()),
                                   "density_linf_max": float(errs[:, 0, 1].max()),
                                   "energy_l1_max": float(errs[:, 1, 0].max()),
-                                  "energy_linf_max": float(errs[:, 1, 1].max())}
+                                  "energy_linf_max": float(errs[:, 1, 1].max()),
+                                  # per run, in request order: relative L1 and
+                                  # L-infinity of density and of energy
+                                  "per_run": [[float(f"{x:.3g}") for x in e.ravel()]
+                                              for e in errs]}
     return res
 
 
