@@ -352,6 +352,15 @@ extern "C" int hydro_amd_batch_upload(hydro_amd_batch* b, int id, const double* 
 
 // ---- task tables -------------------------------------------------------------------
 
+// Rows per wave in an instance's y tasks: the instance's rows split evenly
+// over as few row groups (of waves x tune.yseg rows) as cover them, so no
+// group ends with a short segment that pays the full halo.
+static int inst_yseg(const hydro_amd_batch* b, int ny) {
+  const long W = b->tune.waves, per = W * b->tune.yseg;
+  const long nrg = (ny + per - 1) / per;
+  return (int) ((ny + W * nrg - 1) / (W * nrg));
+}
+
 template <typename S>
 static InstDev<S> inst_dev(hydro_amd_batch* b, int id) {
   const Inst& in = b->inst[id];
@@ -371,6 +380,7 @@ static InstDev<S> inst_dev(hydro_amd_batch* b, int id) {
   d.bd = in.p.boundary_down;
   d.bu = in.p.boundary_up;
   d.nseg = (in.p.nx + kXSeg - 1) / kXSeg;
+  d.yseg = inst_yseg(b, in.p.ny);
   return d;
 }
 
@@ -383,7 +393,7 @@ static void tasks_of(const hydro_amd_batch* b, int id, int kind, std::vector<BTa
     for (long a = 0; a < tot; a += per) out.push_back(BTask{id, 1, (int) a, 0});
   } else {
     const int ncb = (in.p.nx + 63) / 64;
-    const long rows = (long) W * b->tune.yseg;
+    const long rows = (long) W * inst_yseg(b, in.p.ny);
     const int nrg = (int) ((in.p.ny + rows - 1) / rows);
     for (int rg = 0; rg < nrg; rg++)
       for (int cb = 0; cb < ncb; cb++) out.push_back(BTask{id, 2, cb, rg});
@@ -637,7 +647,7 @@ extern "C" int hydro_amd_batch_autotune(hydro_amd_batch* b, int reps, int* best)
   BCHK(hipEventCreate(&e1));
   for (int w : {2, 4})
     for (int xi : {2, 4, 8})
-      for (int ys : {16, 32, 64}) {
+      for (int ys : {16, 32, 64, 128}) {
         if (hydro_amd_batch_set_tune(b, w, xi, ys) || rebuild(b)) break;
         for (int r = -1; r < reps; r++) {  // r = -1 warms up
           if (r == 0) (void) hipEventRecord(e0, b->st);

@@ -12,7 +12,8 @@
 // the GPU, and the workgroups pull task indices from an atomic counter.  A
 // task is one workgroup's worth of one instance:
 //   x task: wpb * xiter consecutive 60-cell row segments (each wave walks xiter)
-//   y task: 64 columns x (wpb * yseg) rows (each wave marches yseg rows)
+//   y task: 64 columns x (wpb * yseg) rows (each wave marches yseg rows; the
+//           instance's yseg evens its rows out over its row groups)
 // so the CFL reduction of a task never mixes instances.  Run control (t, dt,
 // nstep, active) and CFL slots are per instance, on the device.
 #include "hydro_batch.hpp"
@@ -89,10 +90,10 @@ __global__ __launch_bounds__(256) void k_batch_sweep_y(const InstDev<S>* __restr
     const int col = tk.a * 64 + (threadIdx.x & 63);
     const bool colok = col < d.g.nx;
     const long i = 2 + (colok ? col : d.g.nx - 1);
-    const int j0 = 2 + (tk.b * wpb + wave) * yseg;
+    const int j0 = 2 + (tk.b * wpb + wave) * d.yseg;
     double cx = 0.0, cy = 0.0;
     if (j0 < d.g.ny + 2) {  // wave-uniform
-      const int j1 = min(j0 + yseg, d.g.ny + 2);
+      const int j1 = min(j0 + d.yseg, d.g.ny + 2);
       y_march<S, ST1, false>(d.g, d.ph, tr, dtdx, in, out, colok, i, j0, j1, cx, cy);
     }
   }

@@ -22,6 +22,7 @@ struct InstDev {
   uint64_t* cour;         // 2 * kInstSlots
   int bl, br, bd, bu;     // boundary types
   int nseg;               // 60-cell segments per row
+  int yseg;               // rows each wave marches in this instance's y tasks
 };
 
 // One workgroup's work: kind 1 = x (a = first segment), 2 = y (a = 64-column
@@ -33,7 +34,8 @@ struct BTask {
 struct BatchTune {
   int waves;  // waves per workgroup (1..4)
   int xiter;  // segments each wave walks in an x task
-  int yseg;   // rows each wave marches in a y task
+  int yseg;   // at most this many rows per wave in a y task (each instance
+              // splits its rows evenly into row groups of waves x yseg)
 };
 
 // kind 1: x tasks, 2: y tasks (a phase launches both, on their own tables).
