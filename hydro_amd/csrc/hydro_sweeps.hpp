@@ -24,29 +24,28 @@ namespace hydro {
 
 // ---- lane exchange -----------------------------------------------------------
 
-// lane l receives the value of lane l-1 (DPP wave_shr:1); lane 0 keeps its own.
+// lane l receives the value of lane l-1 (DPP wave_ror:1, so lane 0 receives
+// lane 63's); from_right: lane l+1's (wave_rol:1, lane 63 receives lane 0's).
+// Every source lane is valid, so each 32-bit half is one v_mov_b32_dpp into a
+// fresh register.  The wrapped values reach only lanes 0-1 and 62-63 of a
+// segment, which are halo lanes whose results are never stored.
 __device__ __forceinline__ double from_left(double x) {
   int lo = __double2loint(x), hi = __double2hiint(x);
-  lo = __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false);
-  hi = __builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false);
+  lo = __builtin_amdgcn_mov_dpp(lo, 0x13C, 0xf, 0xf, false);
+  hi = __builtin_amdgcn_mov_dpp(hi, 0x13C, 0xf, 0xf, false);
   return __hiloint2double(hi, lo);
 }
-// lane l receives the value of lane l+1 (DPP wave_shl:1); lane 63 keeps its own.
 __device__ __forceinline__ double from_right(double x) {
   int lo = __double2loint(x), hi = __double2hiint(x);
-  lo = __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false);
-  hi = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
+  lo = __builtin_amdgcn_mov_dpp(lo, 0x134, 0xf, 0xf, false);
+  hi = __builtin_amdgcn_mov_dpp(hi, 0x134, 0xf, 0xf, false);
   return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ float from_left(float x) {
-  int b = __float_as_int(x);
-  b = __builtin_amdgcn_update_dpp(b, b, 0x138, 0xf, 0xf, false);
-  return __int_as_float(b);
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x13C, 0xf, 0xf, false));
 }
 __device__ __forceinline__ float from_right(float x) {
-  int b = __float_as_int(x);
-  b = __builtin_amdgcn_update_dpp(b, b, 0x130, 0xf, 0xf, false);
-  return __int_as_float(b);
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x134, 0xf, 0xf, false));
 }
 template <typename R>
 __device__ __forceinline__ Q4<R> from_left4(const Q4<R>& a) {
