@@ -131,6 +131,8 @@ def lib() -> C.CDLL:
                        ("hydro_amd_grid_init_blast", [G, C.c_int, C.c_int, C.c_double]),
                        ("hydro_amd_grid_upload", [G, _dp]),
                        ("hydro_amd_grid_download", [G, _dp]),
+                       ("hydro_amd_grid_image", [G, C.c_int, C.c_void_p, C.c_longlong,
+                                                 C.POINTER(C.c_int), C.POINTER(C.c_int)]),
                        ("hydro_amd_grid_godunov", [G, C.c_int, C.c_double]),
                        ("hydro_amd_grid_make_boundary", [G, C.c_int]),
                        ("hydro_amd_grid_compute_deltat", [G, C.POINTER(C.c_double)]),
@@ -256,6 +258,16 @@ class Grid:
         u = np.empty(self.p.shape())
         _check(lib().hydro_amd_grid_download(self._h, u))
         return u
+
+    def image(self, shrink_size: int = 1000) -> np.ndarray:
+        """Density image of the current state, computed on the GPU
+        (HydroC99_2Domp/Src/Image.c:111-166): uint8 [h, w, 3]."""
+        w, h = C.c_int(), C.c_int()
+        _check(lib().hydro_amd_grid_image(self._h, shrink_size, None, 0, C.byref(w), C.byref(h)))
+        out = np.zeros(max(3 * w.value * h.value, 1), np.uint8)
+        _check(lib().hydro_amd_grid_image(self._h, shrink_size, out.ctypes.data, out.size,
+                                          C.byref(w), C.byref(h)))
+        return out[:3 * w.value * h.value].reshape(h.value, w.value, 3)
 
     # the reference call surface on this grid
     def godunov(self, idim: int, dt: float) -> None:

@@ -371,6 +371,37 @@ extern "C" int hydro_amd_grid_download(hydro_amd_grid* g, double* u) {
   return stage_d2h(u, 8, src, (int) g->esz, g->geom.pitch, nxt, rows, g->st) ? -1 : 0;
 }
 
+// Image size rule of HydroC99_2Domp/Src/Image.c:124-126 (single rank): the
+// smallest shrink with nx/shrink and ny/shrink both <= shrink_size.
+extern "C" int hydro_amd_grid_image(hydro_amd_grid* g, int shrink_size, unsigned char* rgb,
+                                    long long cap, int* width, int* height) {
+  if (!g || !width || !height) return fail("null argument");
+  if (shrink_size < 1) return fail("shrink_size %d: must be at least 1", shrink_size);
+  int s = 1;
+  while (g->p.nx / s > shrink_size || g->p.ny / s > shrink_size) s++;
+  const int w = g->p.nx / s, h = g->p.ny / s;
+  *width = w;
+  *height = h;
+  if (!rgb) return 0;
+  const long long need = 3LL * w * h;
+  if (cap < need) return fail("image buffer of %lld bytes, %lld needed", cap, need);
+  if (need == 0) return 0;
+  void* d = nullptr;  // [8-byte maximum key][pixels]
+  HIPCHK(hipMallocAsync(&d, (size_t) (8 + need), g->st));
+  unsigned long long* key = (unsigned long long*) d;
+  unsigned char* px = (unsigned char*) d + 8;
+  if (g->prec == HYDRO_AMD_FP64)
+    launch_image<double>(g->geom, g->vd[0].p[0], s, w, h, key, px, g->st);
+  else
+    launch_image<float>(g->geom, g->vf[0].p[0], s, w, h, key, px, g->st);
+  const hipError_t e1 = hipMemcpyAsync(rgb, px, (size_t) need, hipMemcpyDeviceToHost, g->st);
+  const hipError_t e2 = hipFreeAsync(d, g->st);
+  const hipError_t e3 = hipStreamSynchronize(g->st);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+    return fail("image: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3));
+  return 0;
+}
+
 // ---- steps ---------------------------------------------------------------------------
 
 static void enq_boundary(hydro_amd_grid* g, int idim, int src, const RunCtl* ctl) {

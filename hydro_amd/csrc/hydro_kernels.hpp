@@ -87,6 +87,13 @@ void launch_init(const GridGeom& g, GridView<S> a, GridView<S> b, int bx, int by
 
 void launch_copy_probe(const double* a, double* b, long n, hipStream_t st);
 
+// Density image (hydro_image.hip): clears *key, reduces the maximum density
+// of the padded plane `rho` into it, then writes w*h RGB pixels (3 bytes
+// each, row cy = 0 first) sampled every `shrink` interior cells.
+template <typename S>
+void launch_image(const GridGeom& g, const S* rho, int shrink, int w, int h,
+                  unsigned long long* key, unsigned char* rgb, hipStream_t st);
+
 // Pitched device rows -> contiguous host rows through pinned staging buffers
 // (hydro_stage.cpp); dst_esz == src_esz, or 8 with src_esz == 4 (widening).
 int stage_d2h(void* dst, int dst_esz, const void* src, int src_esz, long src_pitch, long width,

@@ -8,7 +8,11 @@
  * HydroC99_2DMpi/Src/main.c:244-364 does (dt on even steps, halved at step 0,
  * hydro_godunov(1) on even steps and (2) on odd ones, t += dt, VTK output on
  * noutput / dtoutput, the "--> step=" line with MC/s, the average MC/s over
- * steps > 5).  -native runs the same loop on the GPU (hydro_amd_grid_run) in
+ * steps > 5).  Density images follow HydroC99_2Domp/Src/main.c:212-221,
+ * 462-479: with `nimage` or `dtimage` in the deck, IMAGE_000000.ppm of the
+ * initial state, then one image every nimage steps (or each time t passes
+ * the next multiple of dtimage), marked " <nnnn>" on the step line after
+ * the VTK marker.  -native runs the same loop on the GPU (hydro_amd_grid_run) in
  * chunks of K steps, with one host round trip per chunk.
  *
  * -dtlog writes "step t dt" per step with %.17g; -dump writes the final state
@@ -108,6 +112,15 @@ int main(int argc, char **argv) {
     while (next_output_time <= H.t) next_output_time += H.dtoutput;  /* after a restart */
   }
   if (H.dtoutput > 0 || H.noutput > 0) vtkfile(++nvtk, H, &Hv);
+  int nimage = 0, nimg = 0, time_image = 0;
+  double dtimage = 0, next_image_time = 0;
+  hydro_amd_image_cadence(&nimage, &dtimage);
+  if (dtimage > 0) {
+    time_image = 1;
+    next_image_time += dtimage;
+    while (next_image_time <= H.t) next_image_time += dtimage;  /* after a restart */
+  }
+  if (dtimage > 0 || nimage > 0) DumpImage(nimg++, &H, &Hv);
   fprintf(stdout, "Hydro starts main loop.\n");
   allocate_work_space(H.nxyt, H, &Hw_godunov, &Hvw_godunov);
   compute_deltat_init_mem(H, &Hw_deltat, &Hvw_deltat);
@@ -140,7 +153,9 @@ int main(int argc, char **argv) {
     } else {
       int want = chunk;
       if (H.noutput > 0 && !time_output) want = H.noutput - (H.nstep % H.noutput);
-      if (time_output) want = 1;  /* output times are only known step by step */
+      if (nimage > 0 && !time_image && nimage - (H.nstep % nimage) < want)
+        want = nimage - (H.nstep % nimage);
+      if (time_output || time_image) want = 1;  /* output times are only known step by step */
       if (want > H.nstepmax - H.nstep) want = H.nstepmax - H.nstep;
       chunk_dt = (double *) malloc(sizeof(double) * (size_t) want);
       hydro_amd_grid_set_limits(g, H.tend, H.nstepmax);
@@ -171,6 +186,16 @@ int main(int argc, char **argv) {
         vtkfile(++nvtk, H, &Hv);
         next_output_time += H.dtoutput;
         snprintf(out, sizeof(out), " [%04d]", nvtk);
+      }
+      if (time_image == 0 && nimage > 0) {
+        if ((H.nstep % nimage) == 0) {
+          DumpImage(nimg, &H, &Hv);
+          snprintf(out + strlen(out), sizeof(out) - strlen(out), " <%04d>", nimg++);
+        }
+      } else if (time_image == 1 && H.t >= next_image_time) {
+        DumpImage(nimg, &H, &Hv);
+        next_image_time += dtimage;
+        snprintf(out + strlen(out), sizeof(out) - strlen(out), " <%04d>", nimg++);
       }
       fprintf(stdout, "--> step=%4d, %12.5e, %10.5e %.3lf MC/s%s\n", H.nstep, H.t, dt, mcs, out);
       if (H.nstep > 5) {

@@ -10,7 +10,10 @@
  *    scheme one of muscl/plmde/collela;
  *  - single-rank geometry (:372-377) and the nxystep default/clamp (:430-436);
  *  - the run summary box (:438-451).
- * One key is added: `gamma` (the reference hard-wires 1.4).
+ * One key is added: `gamma` (the reference hard-wires 1.4).  The image
+ * cadence keys of HydroC99_2Domp (`nimage` %d, `dtimage` %lf,
+ * HydroC99_2Domp/Src/parametres.c:165-166, 202-203) are read too and kept
+ * outside hydroparam_t, which has no fields for them.
  */
 #include <limits.h>
 #include <stddef.h>
@@ -18,6 +21,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "../../include/hydro_amd.h"
 #include "../../include/hydro_surface.h"
 
 static void usage(void) {
@@ -112,6 +116,15 @@ static const struct real_key real_keys[] = {
     {"gamma", offsetof(hydroparam_t, gamma)},
 };
 
+/* nimage / dtimage of the last deck read (0 / 0.0: no images). */
+static int g_nimage = 0;
+static double g_dtimage = 0.0;
+
+void hydro_amd_image_cadence(int *nimage, double *dtimage) {
+  if (nimage) *nimage = g_nimage;
+  if (dtimage) *dtimage = g_dtimage;
+}
+
 static void read_deck(const char *path, hydroparam_t *H) {
   FILE *fd = fopen(path, "r");
   char line[1024];
@@ -134,6 +147,14 @@ static void read_deck(const char *path, hydroparam_t *H) {
         sscanf(val, "%lf", (double *) ((char *) H + real_keys[k].off));
         done = 1;
       }
+    if (!done && strcmp(key, "nimage") == 0) {
+      sscanf(val, "%d", &g_nimage);
+      done = 1;
+    }
+    if (!done && strcmp(key, "dtimage") == 0) {
+      sscanf(val, "%lf", &g_dtimage);
+      done = 1;
+    }
     if (!done && strcmp(key, "scheme") == 0) {
       if (strcmp(val, "muscl") == 0) H->scheme = HSCHEME_MUSCL;
       else if (strcmp(val, "plmde") == 0) H->scheme = HSCHEME_PLMDE;
@@ -169,6 +190,8 @@ void process_args(int argc, char **argv, hydroparam_t *H) {
     fprintf(stderr, "Option -i is missing\n");
     exit(1);
   }
+  g_nimage = 0;
+  g_dtimage = 0.0;
   read_deck(deck, H);
 
   H->globnx = H->nx;

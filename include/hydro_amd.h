@@ -59,6 +59,14 @@ int hydro_amd_grid_init_blast(hydro_amd_grid *g, int bx, int by, double e);
 /* Whole-array copies in the reference layout (FP32 storage converts). */
 int hydro_amd_grid_upload(hydro_amd_grid *g, const double *uold);
 int hydro_amd_grid_download(hydro_amd_grid *g, double *uold);
+/* Density image of the current state, HydroC99_2Domp/Src/Image.c:111-166 for
+ * one rank: shrink = the smallest integer with nx/shrink and ny/shrink both
+ * <= shrink_size; *width = nx/shrink, *height = ny/shrink; pixel (cx, cy) is
+ * interior cell (cx*shrink, cy*shrink), rows cy = 0 first, 3 equal bytes
+ * (uint8_t) fabs(255 * rho / max), max over every padded cell.  With rgb NULL
+ * only the size is returned; otherwise cap must hold 3*w*h bytes. */
+int hydro_amd_grid_image(hydro_amd_grid *g, int shrink_size, unsigned char *rgb, long long cap,
+                         int *width, int *height);
 /* hydro_godunov (hydro_godunov.c:63), make_boundary (make_boundary.c:146)
  * and compute_deltat (compute_deltat.c:166) on one grid. */
 int hydro_amd_grid_godunov(hydro_amd_grid *g, int idim, double dt);
@@ -203,6 +211,12 @@ int hydro_amd_checkpoint(const char *path, const hydroparam_t H, hydrovar_t *Hv,
  * dt an odd step reuses); refreshes the GPU copy.  0, or -1 with
  * hydro_amd_last_error() set. */
 int hydro_amd_restore(const char *path, hydroparam_t *H, hydrovar_t *Hv, double *dt);
+
+/* Image cadence keys of the last deck process_args() read (HydroC99_2Domp/
+ * Src/parametres.c:69-70, 165-166, 202-203: `nimage` every n steps, or
+ * `dtimage` in simulated time; both 0 by default).  hydroparam_t of
+ * HydroC99_2DMpi has no fields for them, so they are kept beside it. */
+void hydro_amd_image_cadence(int *nimage, double *dtimage);
 
 #ifdef __cplusplus
 }

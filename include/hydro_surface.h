@@ -138,6 +138,14 @@ void make_boundary(int idim, const hydroparam_t H, hydrovar_t *Hv);
 /* HydroC99_2DMpi/Src/vtkfile.h:39 -- Dep/NNNN/NN/Hydro_RRRRR_SSSS.vtr + .pvtr
  * + Hydro.pvd, base64 Float32 cell data, the reference's format */
 void vtkfile(int step, const hydroparam_t H, hydrovar_t *Hv);
+/* HydroC99_2Domp/Src/Image.h:30 (the OpenMP variant's image output, its
+ * default PPM build): writes IMAGE_nnnnnn.ppm (prefixed by $HYDROC_IMG_PREFIX
+ * when set), "P6\n<w> <h>\n255\n", w*h grey RGB pixels (uint8_t)
+ * fabs(255 * rho / max rho), then "#EOF\n"; the image is at most 1000
+ * pixels a side (Image.c:124-126, shrinkSize 1000 of parametres.c:68).
+ * Computed on the GPU from the device state; aborts when the file cannot be
+ * opened, as Image.c:298-310. */
+void DumpImage(int n, hydroparam_t *H, hydrovar_t *Hv);
 
 #ifdef __cplusplus
 }

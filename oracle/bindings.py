@@ -135,6 +135,9 @@ class Port:
         lib.ho_riemann.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _ip, C.c_double, C.c_double,
                                    C.c_double, C.c_int]
         lib.ho_cmpflx.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double]
+        lib.ho_image.argtypes = [C.c_int, C.c_int, _dp, C.c_int, C.c_void_p,
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        lib.ho_image.restype = C.c_int
         self.lib = lib
 
     # whole grid
@@ -165,6 +168,16 @@ class Port:
         k = self.lib.ho_run(_ho(p), u, p.tend, p.nstepmax, max_steps, C.byref(ct), C.byref(cn),
                             C.byref(cdt), log)
         return log[:k].copy(), (ct.value, cn.value, cdt.value)
+
+    def image(self, u: np.ndarray, shrink_size: int = 1000) -> np.ndarray:
+        """Density image of a [4, ny+4, nx+4] state: uint8 [h, w, 3]."""
+        nx, ny = u.shape[2] - 4, u.shape[1] - 4
+        rho = np.ascontiguousarray(u[0], dtype=np.float64)
+        w, h = C.c_int(), C.c_int()
+        self.lib.ho_image(nx, ny, rho, shrink_size, None, C.byref(w), C.byref(h))
+        out = np.zeros(max(3 * w.value * h.value, 1), np.uint8)
+        self.lib.ho_image(nx, ny, rho, shrink_size, out.ctypes.data, C.byref(w), C.byref(h))
+        return out[:3 * w.value * h.value].reshape(h.value, w.value, 3)
 
     # stages over one line, SoA [4][n]
     def constoprim(self, u, smallr):

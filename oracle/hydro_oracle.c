@@ -430,3 +430,35 @@ int ho_run(const ho_params *p, double *uold, double tend, int nstepmax, int max_
   }
   return done;
 }
+
+int ho_image(int nx, int ny, const double *uold, int shrink_size, unsigned char *rgb, int *w,
+             int *h) {
+  /* HydroC99_2Domp/Src/Image.c:111-166 for one rank: image size (:124-131),
+   * the maximum density over every padded cell scanned from cell (0, 0) with
+   * `v > max` (getMaxVarValues, :71-110), then grey pixels of interior cells
+   * (2 + cx*shrink, 2 + cy*shrink).  The byte is the reference's own C
+   * expression, so this compiles to the same conversion as the reference's
+   * gcc build (cvttsd2si to int32, low byte). */
+  const int nxt = nx + 4, nyt = ny + 4;
+  int shrink = 1;
+  while ((nx / shrink) > shrink_size || (ny / shrink) > shrink_size) shrink++;
+  *w = nx / shrink;
+  *h = ny / shrink;
+  if (!rgb) return shrink;
+  double mx = uold[0];
+  for (int y = 0; y < nyt; y++)
+    for (int x = 0; x < nxt; x++) {
+      double v = uold[(long) y * nxt + x];
+      if (v > mx) mx = v;
+    }
+  unsigned char *ptr = rgb;
+  for (int cy = 0; cy < *h; cy++)
+    for (int cx = 0; cx < *w; cx++) {
+      int x = 2 + cx * shrink, y = 2 + cy * shrink;
+      unsigned char vd = (unsigned char) fabs(255 * uold[(long) y * nxt + x] / mx);
+      *(ptr++) = vd;
+      *(ptr++) = vd;
+      *(ptr++) = vd;
+    }
+  return shrink;
+}

@@ -64,6 +64,11 @@ void ho_godunov(int idim_start, double dt, const ho_params *p, double *uold);
 int ho_run(const ho_params *p, double *uold, double tend, int nstepmax, int max_steps,
            double *t, int *nstep, double *dt, double *dt_log);
 
+/* density image of HydroC99_2Domp/Src/Image.c:111-166 (one rank): returns
+ * the shrink factor, sets *w, *h; fills rgb (3*w*h bytes) unless it is NULL */
+int ho_image(int nx, int ny, const double *uold, int shrink_size, unsigned char *rgb, int *w,
+             int *h);
+
 #ifdef __cplusplus
 }
 #endif

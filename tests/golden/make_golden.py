@@ -5,7 +5,8 @@ Run in a container that holds the reference sources (they are compiled in
 place into oracle/_ref by `make -C oracle`); the vectors are data only and are
 committed, the reference never travels.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            (everything)
+    python tests/golden/make_golden.py images     (images.npz only)
 
 Files written:
   stages.npz      per-stage inputs/outputs of the reference's own constoprim,
@@ -15,6 +16,10 @@ Files written:
   traj_*.npz      time-loop runs of the reference (dt and t per step, sha256 of
                   the full uold after every step, final uold)
   vtk_ref/        the reference vtkfile() output for a small state
+  images.npz      PPM density images written by the reference OpenMP
+                  executable (HydroC99_2Domp, built in place as
+                  oracle/_ref/hydro_omp) for three decks, with the decks and
+                  the step -> image markers of its step lines
   decks/          copies of the reference's input decks (data files)
   MANIFEST.json   file list with sha256
 """
@@ -173,7 +178,68 @@ def blast_state(p: Params, bx, by, e):
     return port.init_blast(p, bx, by, e)
 
 
+# ---------------------------------------------------------------------------
+# images of the reference OpenMP executable
+
+IMAGE_DECKS = {
+    # 250x125 corner blast, an image every 10 steps
+    "corner_n10": dict(nx=250, ny=125, dx=0.004, testcase=1, nstepmax=20, extra="nimage=10"),
+    # 1100x60 centred blast: shrink 2 (550x30 pixels), an image every 2 steps
+    "shrink2_n2": dict(nx=1100, ny=60, dx=1.0 / 1100, testcase=0, nstepmax=4, extra="nimage=2"),
+    # 250x125 corner blast, an image each time t passes a multiple of 1e-4
+    "corner_dt": dict(nx=250, ny=125, dx=0.004, testcase=1, nstepmax=20, extra="dtimage=1e-4"),
+}
+
+
+def image_deck(d):
+    return ("&RUN\ntend=1e30\nnstepmax=%d\n%s\n/\n&MESH\nnx=%d\nny=%d\ndx=%r\n"
+            "boundary_left=1\nboundary_right=1\nboundary_down=1\nboundary_up=1\ntestcase=%d\n/\n"
+            "&HYDRO\ncourant_factor=0.8\nniter_riemann=10\niorder=2\nslope_type=1.\n"
+            "scheme=muscl\nsmallr=1e-10\nsmallc=1e-10\n/\n"
+            % (d["nstepmax"], d["extra"], d["nx"], d["ny"], d["dx"], d["testcase"]))
+
+
+def images():
+    import re
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "hydro_omp")
+    out = {}
+    for name, d in IMAGE_DECKS.items():
+        deck = image_deck(d)
+        with tempfile.TemporaryDirectory() as tmp:
+            with open(os.path.join(tmp, "deck.nml"), "w") as f:
+                f.write(deck)
+            log = subprocess.run([exe, "-i", "deck.nml"], cwd=tmp, capture_output=True, text=True,
+                                 env=dict(os.environ, OMP_NUM_THREADS="4"), check=True).stdout
+            files = sorted(f for f in os.listdir(tmp) if f.startswith("IMAGE_"))
+            for f in files:
+                out[f"{name}/{f}"] = np.frombuffer(open(os.path.join(tmp, f), "rb").read(), np.uint8)
+        marks = [(int(a), int(b)) for a, b in re.findall(r"--> step=\s*(\d+),.*<(\d+)>", log)]
+        out[f"{name}/deck"] = np.frombuffer(deck.encode(), np.uint8)
+        out[f"{name}/marks"] = np.array(marks, dtype=np.int64).reshape(-1, 2)
+    np.savez_compressed(os.path.join(HERE, "images.npz"), **out)
+
+
+def manifest():
+    man = {}
+    for dp, _, fs in os.walk(HERE):
+        for f in fs:
+            if f.endswith((".py", ".pyc")) or f == "MANIFEST.json":
+                continue
+            path = os.path.join(dp, f)
+            man[os.path.relpath(path, HERE)] = hashlib.sha256(open(path, "rb").read()).hexdigest()
+    with open(os.path.join(HERE, "MANIFEST.json"), "w") as fh:
+        json.dump(dict(sorted(man.items())), fh, indent=1)
+    print("golden vectors written:", len(man), "files")
+
+
 def main():
+    if sys.argv[1:] == ["images"]:
+        oracle.build(ref=True)
+        images()
+        manifest()
+        return
     oracle.build(ref=True)
     ref = oracle.Ref()
     stage_vectors(ref)
@@ -235,16 +301,8 @@ def main():
     for f in sorted(os.listdir(REF_INPUT)):
         shutil.copy(os.path.join(REF_INPUT, f), os.path.join(ddir, f))
     shutil.copy(REF_DECK6000, os.path.join(ddir, "input6000.nml"))
-    man = {}
-    for dp, _, fs in os.walk(HERE):
-        for f in fs:
-            if f.endswith((".py", ".pyc")) or f == "MANIFEST.json":
-                continue
-            path = os.path.join(dp, f)
-            man[os.path.relpath(path, HERE)] = hashlib.sha256(open(path, "rb").read()).hexdigest()
-    with open(os.path.join(HERE, "MANIFEST.json"), "w") as fh:
-        json.dump(dict(sorted(man.items())), fh, indent=1)
-    print("golden vectors written:", len(man), "files")
+    images()
+    manifest()
 
 
 if __name__ == "__main__":
