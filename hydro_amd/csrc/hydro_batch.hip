@@ -16,6 +16,10 @@
 //           instance's yseg evens its rows out over its row groups)
 // so the CFL reduction of a task never mixes instances.  Run control (t, dt,
 // nstep, active) and CFL slots are per instance, on the device.
+//
+// Compiled twice, as hydro_kernels.hip: the FP64 unit (-ffp-contract=off)
+// and the mixed-precision unit (HYDRO_MIXED_UNIT, -ffp-contract=on) with
+// only the FP32-storage sweeps and CFL kernels.
 #include "hydro_batch.hpp"
 #include "hydro_sweeps.hpp"
 
@@ -182,6 +186,9 @@ __global__ void k_batch_ctl_end(const InstDev<S>* inst, const int* ids, int nids
   c->t += c->dt;
 }
 
+static inline unsigned cdivu(long a, long b) { return (unsigned) ((a + b - 1) / b); }
+
+#ifndef HYDRO_MIXED_UNIT
 __global__ void k_batch_set_ctl(RunCtl* c, double t, double dt, double tend, int nstep,
                                 int nstepmax, int active) {
   if (threadIdx.x != 0) return;
@@ -203,8 +210,6 @@ __global__ void k_fill_u64(uint64_t* p, long n) {
     p[k] = 0;
 }
 
-static inline unsigned cdivu(long a, long b) { return (unsigned) ((a + b - 1) / b); }
-
 void launch_batch_set_ctl(RunCtl* c, double t, double dt, double tend, int nstep, int nstepmax,
                           int active, hipStream_t st) {
   hipLaunchKernelGGL(k_batch_set_ctl, dim3(1), dim3(64), 0, st, c, t, dt, tend, nstep, nstepmax,
@@ -217,6 +222,7 @@ void launch_zero_u64(uint64_t* p, long n, hipStream_t st) {
   if (nb > 1024) nb = 1024;
   hipLaunchKernelGGL(k_fill_u64, dim3((unsigned) nb), dim3(256), 0, st, p, n);
 }
+#endif  // HYDRO_MIXED_UNIT
 
 template <typename S>
 int batch_sweep_occupancy(int kind, int waves) {
@@ -228,8 +234,11 @@ int batch_sweep_occupancy(int kind, int waves) {
   if (e != hipSuccess) return 2;
   return nb > 0 ? nb : 1;
 }
+#ifndef HYDRO_MIXED_UNIT
 template int batch_sweep_occupancy<double>(int, int);
+#else
 template int batch_sweep_occupancy<float>(int, int);
+#endif
 
 template <typename S, bool ST1>
 static void launch_sweep_part(int kind, const InstDev<S>* inst, const BTask* tasks, int ntask,
@@ -283,16 +292,23 @@ void launch_batch_ctl(const InstDev<S>* inst, const int* ids, int nids, bool beg
   }
 }
 
-#define INST(S)                                                                                  \
+#define INST_ARITH(S)                                                                            \
   template void launch_batch_sweep<S>(int, const InstDev<S>*, const BTask*, int, int, int,      \
                                       const BatchTune&, int, uint64_t*, hipStream_t);             \
+  template void launch_batch_courant<S>(const InstDev<S>*, const int*, int, hipStream_t);
+#define INST_MOVE(S)                                                                             \
   template void launch_batch_boundary<S>(const InstDev<S>*, const BTask*, int, int, int,        \
                                          hipStream_t);                                           \
-  template void launch_batch_courant<S>(const InstDev<S>*, const int*, int, hipStream_t);       \
   template void launch_batch_ctl<S>(const InstDev<S>*, const int*, int, bool, uint64_t*,      \
                                     hipStream_t);
-INST(double)
-INST(float)
-#undef INST
+#ifndef HYDRO_MIXED_UNIT
+INST_ARITH(double)
+INST_MOVE(double)
+INST_MOVE(float)
+#else
+INST_ARITH(float)
+#endif
+#undef INST_ARITH
+#undef INST_MOVE
 
 }  // namespace hydro

@@ -2,6 +2,15 @@
 // HydroC99's split Godunov/MUSCL step, ghost fill, CFL reduction, initial
 // state and the device-side time-loop control.  The per-task bodies live in
 // hydro_sweeps.hpp (shared with the batched kernels).
+//
+// The file is compiled twice.  The FP64 unit (the default) holds everything
+// except the FP32-storage sweeps and CFL kernels and is built with
+// -ffp-contract=off, so the FP64 path is op-for-op the reference's.  The
+// mixed-precision unit (HYDRO_MIXED_UNIT defined) holds only those FP32
+// kernels and is built with -ffp-contract=on: the mixed form carries no
+// bitwise contract with the reference, and multiply-adds fused within each
+// source expression cut its VALU instruction count while keeping the
+// single-grid and batched kernels (same source) bit-identical.
 #include "hydro_kernels.hpp"
 #include "hydro_sweeps.hpp"
 
@@ -100,6 +109,7 @@ __global__ __launch_bounds__(256) void k_boundary(int idim, GridGeom g, int bl, 
   boundary_line<S>(idim, g, bl, br, bd, bu, src.p[v], has_dst2 ? dst2.p[v] : nullptr, v, k);
 }
 
+#ifndef HYDRO_MIXED_UNIT
 // ---- dt and run control ------------------------------------------------------
 
 __global__ __launch_bounds__(64) void k_dt_finalize(uint64_t* cour, double cf, double dx,
@@ -139,6 +149,7 @@ __global__ void k_ctl_end(RunCtl* ctl) {
 __global__ void k_clear_slots(uint64_t* cour) {
   for (int s = threadIdx.x; s < 2 * kCourSlots; s += blockDim.x) cour[s] = 0;
 }
+#endif  // HYDRO_MIXED_UNIT
 
 template <typename S>
 __global__ __launch_bounds__(256) void k_init(GridGeom g, GridView<S> a, GridView<S> b, int bx,
@@ -148,6 +159,7 @@ __global__ __launch_bounds__(256) void k_init(GridGeom g, GridView<S> a, GridVie
   init_cell<S>(g, a, b, t, bx, by0, by1, eb);
 }
 
+#ifndef HYDRO_MIXED_UNIT
 // ---- plain copy (measurement reference) ---------------------------------------
 
 // Streams n doubles from a to b; used to calibrate HBM byte counters and as
@@ -162,6 +174,7 @@ __global__ __launch_bounds__(256) void k_copy_probe(const double* __restrict__ a
 void launch_copy_probe(const double* a, double* b, long n, hipStream_t st) {
   hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, st, a, b, n);
 }
+#endif  // HYDRO_MIXED_UNIT
 
 // ---- launchers -----------------------------------------------------------------
 
@@ -231,6 +244,7 @@ void launch_courant(const GridGeom& g, const PhysArgs& pa, GridView<S> in, uint6
   hipLaunchKernelGGL(k_courant<S>, dim3((unsigned) nb), dim3(256), 0, st, g, ph, in, cour, ctl);
 }
 
+#ifndef HYDRO_MIXED_UNIT
 void launch_dt_finalize(uint64_t* cour, double cf, double dx, double smallc, double* dt_out,
                         hipStream_t st) {
   hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, cour, cf, dx, smallc, dt_out);
@@ -248,6 +262,7 @@ void launch_ctl_end(RunCtl* ctl, hipStream_t st) {
 void launch_clear_slots(uint64_t* cour, hipStream_t st) {
   hipLaunchKernelGGL(k_clear_slots, dim3(1), dim3(256), 0, st, cour);
 }
+#endif  // HYDRO_MIXED_UNIT
 
 template <typename S>
 void launch_init(const GridGeom& g, GridView<S> a, GridView<S> b, int bx, int by0, int by1,
@@ -256,11 +271,11 @@ void launch_init(const GridGeom& g, GridView<S> a, GridView<S> b, int bx, int by
   hipLaunchKernelGGL(k_init<S>, dim3(cdiv(n, 256)), dim3(256), 0, st, g, a, b, bx, by0, by1,
                      (S) e_blast);
 }
+#ifndef HYDRO_MIXED_UNIT
 template void launch_init<double>(const GridGeom&, GridView<double>, GridView<double>, int, int,
                                   int, double, hipStream_t);
 template void launch_init<float>(const GridGeom&, GridView<float>, GridView<float>, int, int,
                                  int, double, hipStream_t);
-
 template void launch_boundary<double>(int, const GridGeom&, const PhysArgs&, GridView<double>,
                                       GridView<double>, const RunCtl*, hipStream_t);
 template void launch_boundary<float>(int, const GridGeom&, const PhysArgs&, GridView<float>,
@@ -268,12 +283,14 @@ template void launch_boundary<float>(int, const GridGeom&, const PhysArgs&, Grid
 template void launch_sweep<double>(int, const GridGeom&, const PhysArgs&, GridView<double>,
                                    GridView<double>, double, const RunCtl*, uint64_t*,
                                    const SweepTune&, hipStream_t);
+template void launch_courant<double>(const GridGeom&, const PhysArgs&, GridView<double>,
+                                     uint64_t*, const RunCtl*, hipStream_t);
+#else
 template void launch_sweep<float>(int, const GridGeom&, const PhysArgs&, GridView<float>,
                                   GridView<float>, double, const RunCtl*, uint64_t*,
                                   const SweepTune&, hipStream_t);
-template void launch_courant<double>(const GridGeom&, const PhysArgs&, GridView<double>,
-                                     uint64_t*, const RunCtl*, hipStream_t);
 template void launch_courant<float>(const GridGeom&, const PhysArgs&, GridView<float>,
                                     uint64_t*, const RunCtl*, hipStream_t);
+#endif
 
 }  // namespace hydro
