@@ -4,8 +4,8 @@
 //
 // * x sweep: one 64-lane wave owns 64 consecutive cells of a row and updates
 //   the middle 60 (kXSeg).  Each lane turns its cell into primitives;
-//   neighbours come from DPP wave shifts (v_mov_b32_dpp wave_shl/wave_shr), so
-//   slope, trace, Riemann solve, flux and update all stay in registers.
+//   neighbours come from DPP wave rotations (v_mov_b32_dpp wave_ror/wave_rol),
+//   so slope, trace, Riemann solve, flux and update all stay in registers.
 // * y sweep: one lane owns one column and marches down a segment of rows with
 //   a three-cell register window (primitives, one carried trace state, one
 //   carried flux), two rows per trip, loading each row one row ahead of its
