@@ -306,7 +306,7 @@ class Grid:
     def set_limits(self, tend: float, nstepmax: int) -> None:
         _check(lib().hydro_amd_grid_set_limits(self._h, tend, nstepmax))
 
-    def set_tune(self, yseg: int = 64, ywaves: int = 1, xwaves: int = 2, xiter: int = 8) -> None:
+    def set_tune(self, yseg: int = 64, ywaves: int = 1, xwaves: int = 4, xiter: int = 8) -> None:
         _check(lib().hydro_amd_grid_set_tune(self._h, yseg, ywaves, xwaves, xiter))
 
     def autotune(self, reps: int = 3) -> dict:

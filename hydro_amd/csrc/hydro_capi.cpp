@@ -79,7 +79,7 @@ struct hydro_amd_grid {
   long state_ver = 1;
   long cour_ver = 0;     // state version whose CFL maxima the slots hold (0: none)
   bool cour_dirty = false;
-  SweepTune tune = {64, 1, 2, 8};  // the autotuner's usual pick at 8192^2
+  SweepTune tune = {64, 1, 4, 8};  // near the autotuner's pick at 8192^2
   // host mirror of the run control, valid between calls
   double t = 0, dt = 0;
   int nstep = 0;
@@ -700,7 +700,11 @@ extern "C" int hydro_amd_grid_profile(hydro_amd_grid* g, double* ms, long long* 
 // Times one sweep launch shape: `reps` launches reading buffer 0 and writing
 // buffer 1's interior (buffer 0, the state, is untouched; buffer 1 is scratch
 // between steps), HIP events on the grid's stream.  Returns ms per launch.
-static double time_sweep(hydro_amd_grid* g, int idim, const SweepTune& t, int reps) {
+// Average time of one sweep launch in direction idim with launch shape t
+// (reading the state, writing the scratch buffer); with cour non-null, the
+// variant that also folds the CFL maxima of its output into cour.
+static double time_sweep(hydro_amd_grid* g, int idim, const SweepTune& t, int reps,
+                         uint64_t* cour = nullptr) {
   hipEvent_t a, b;
   if (hipEventCreate(&a) != hipSuccess) return 1e30;
   if (hipEventCreate(&b) != hipSuccess) {
@@ -709,10 +713,10 @@ static double time_sweep(hydro_amd_grid* g, int idim, const SweepTune& t, int re
   }
   auto launch = [&] {
     if (g->prec == HYDRO_AMD_FP64)
-      launch_sweep<double>(idim, g->geom, g->ph, g->vd[0], g->vd[1], 1e-12, nullptr, nullptr, t,
+      launch_sweep<double>(idim, g->geom, g->ph, g->vd[0], g->vd[1], 1e-12, nullptr, cour, t,
                            g->st);
     else
-      launch_sweep<float>(idim, g->geom, g->ph, g->vf[0], g->vf[1], 1e-12, nullptr, nullptr, t,
+      launch_sweep<float>(idim, g->geom, g->ph, g->vf[0], g->vf[1], 1e-12, nullptr, cour, t,
                           g->st);
   };
   launch();  // warm
@@ -759,28 +763,53 @@ extern "C" int hydro_amd_grid_autotune(hydro_amd_grid* g, int reps, int* best) {
   SweepTune cur = g->tune, bt = g->tune;
   // y sweep: rows per lane (the slab height each column march covers) x
   // waves per workgroup
-  double by = 1e30;
+  // Candidates are timed in two passes, in opposite orders, and scored by
+  // the sum, so a clock that drifts during tuning favours none of them.
+  std::vector<SweepTune> ys;
   for (int yseg : {16, 32, 64, 128, 256, 512}) {
     if (yseg > 4 * g->p.ny && yseg != 16) continue;
     for (int yw : {1, 2, 4}) {
       SweepTune t = cur;
       t.yseg = yseg;
       t.ywaves = yw;
-      const double ms = time_sweep(g, 2, t, reps);
-      if (ms < by) { by = ms; bt.yseg = yseg; bt.ywaves = yw; }
+      ys.push_back(t);
     }
   }
-  // x sweep: waves per workgroup x segments walked per wave
-  double bx = 1e30;
+  std::vector<double> yms(ys.size(), 0.0);
+  for (int pass = 0; pass < 2; pass++)
+    for (size_t k = 0; k < ys.size(); k++) {
+      const size_t c = pass ? ys.size() - 1 - k : k;
+      yms[c] += time_sweep(g, 2, ys[c], reps);
+    }
+  double by = 1e30;
+  for (size_t c = 0; c < ys.size(); c++)
+    if (yms[c] < by) { by = yms[c]; bt.yseg = ys[c].yseg; bt.ywaves = ys[c].ywaves; }
+  // x sweep: waves per workgroup x segments walked per wave.  Two steps run
+  // one plain x sweep and one that folds the CFL maxima (odd steps), so a
+  // shape is scored by the sum; the maxima go to scratch slots, not the
+  // grid's.
+  uint64_t* scour = nullptr;
+  HIPCHK(hipMalloc(&scour, 2 * kCourSlots * sizeof(uint64_t)));
+  std::vector<SweepTune> xs;
   for (int xw : {1, 2, 4}) {
     for (int xi : {1, 2, 4, 8, 16, 32}) {
       SweepTune t = cur;
       t.xwaves = xw;
       t.xiter = xi;
-      const double ms = time_sweep(g, 1, t, reps);
-      if (ms < bx) { bx = ms; bt.xwaves = xw; bt.xiter = xi; }
+      xs.push_back(t);
     }
   }
+  std::vector<double> xms(xs.size(), 0.0);
+  for (int pass = 0; pass < 2; pass++)
+    for (size_t k = 0; k < xs.size(); k++) {
+      const size_t c = pass ? xs.size() - 1 - k : k;
+      xms[c] += time_sweep(g, 1, xs[c], reps) + time_sweep(g, 1, xs[c], reps, scour);
+    }
+  double bx = 1e30;
+  for (size_t c = 0; c < xs.size(); c++)
+    if (xms[c] < bx) { bx = xms[c]; bt.xwaves = xs[c].xwaves; bt.xiter = xs[c].xiter; }
+  (void) hipStreamSynchronize(g->st);
+  (void) hipFree(scour);
   HIPCHK(hipGetLastError());
   g->tune = bt;
   drop_graph(g);

@@ -42,6 +42,30 @@ __global__ __launch_bounds__(256) void k_sweep_x(GridGeom g, Phys<S> ph, double 
   if (COUR) commit_courant(cx, cy, cour, kCourSlots, blockIdx.x % kCourSlots);
 }
 
+// Workgroup form (W = 2 or 4 waves): a workgroup walks xiter consecutive
+// row pieces of 64*W - 4 updated cells, neighbours through LDS.
+template <typename S, bool ST1, bool COUR, int W>
+__global__ __launch_bounds__(64 * W) void k_sweep_xb(GridGeom g, Phys<S> ph, double dx,
+                                                     double dt_val, const RunCtl* ctl,
+                                                     GridView<S> in, GridView<S> out,
+                                                     uint64_t* cour, int nseg, int xiter) {
+  __shared__ XLds<S, W> sh;
+  double dt = dt_val;
+  if (ctl) {
+    if (!ctl->active) return;  // grid-uniform
+    dt = ctl->dt;
+  }
+  const double dtdx = dt / dx;  // hydro_godunov.c:101
+  const TraceK<S> tk = make_tracek<S>((S) dtdx, ph.scheme);
+  double cx = 0.0, cy = 0.0;
+  SegWalk w((int) blockIdx.x * xiter, nseg);
+  for (int it = 0; it < xiter && w.row < g.ny; it++) {  // workgroup-uniform
+    x_block<S, ST1, COUR, W>(g, ph, tk, dtdx, in, out, w.row, w.seg, sh, cx, cy);
+    w.advance(1, nseg);
+  }
+  if (COUR) commit_courant(cx, cy, cour, kCourSlots, blockIdx.x % kCourSlots);
+}
+
 // ---- y sweep ---------------------------------------------------------------
 
 // A workgroup covers 64 columns; its waves take consecutive row segments of
@@ -198,7 +222,22 @@ void launch_sweep(int idim, const GridGeom& g, const PhysArgs& pa, GridView<S> i
                                   pa.iorder, pa.scheme);
   const bool st1 = pa.slope_type == 1.0;
   const bool c = cour != nullptr;
-  if (idim == 1) {
+  if (idim == 1 && (tune.xwaves == 2 || tune.xwaves == 4)) {
+    const int W = tune.xwaves;
+    const int nseg = (int) cdiv(g.nx, 64 * W - 4);
+    const long pieces = (long) nseg * g.ny;
+    int xiter = tune.xiter > 0 ? tune.xiter : 8;
+    while (xiter > 1 && cdiv(pieces, xiter) < 1024) xiter >>= 1;
+    const dim3 grid(cdiv(pieces, xiter)), blk(64 * W);
+#define HXB(A, B, WW) \
+  hipLaunchKernelGGL((k_sweep_xb<S, A, B, WW>), grid, blk, 0, st, g, ph, pa.dx, dt_val, ctl, in, \
+                     out, cour, nseg, xiter)
+#define HXW(A, B) do { if (W == 2) HXB(A, B, 2); else HXB(A, B, 4); } while (0)
+    if (st1) { if (c) HXW(true, true); else HXW(true, false); }
+    else { if (c) HXW(false, true); else HXW(false, false); }
+#undef HXW
+#undef HXB
+  } else if (idim == 1) {
     const int nseg = (int) cdiv(g.nx, kXSeg);
     const long waves = (long) nseg * g.ny;
     const int xw = tune.xwaves > 0 ? tune.xwaves : 4;

@@ -178,6 +178,95 @@ __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
   }
 }
 
+// ---- x sweep, workgroup form: W waves share one row piece through LDS ------
+
+// LDS of one workgroup: three exchange buffers (primitives, right-face states,
+// fluxes), each four planes of 64*W + 2 values.  Slot t + 1 holds thread t's
+// value; slots 0 and 64*W + 1 repeat the end threads' values, so the halo
+// threads read plausible states (their results are never stored).
+template <typename S, int W>
+struct XLds {
+  S q[4][64 * W + 2], x[4][64 * W + 2], f[4][64 * W + 2];
+};
+
+template <typename S, int W>
+__device__ __forceinline__ void lds_put(S (&b)[4][64 * W + 2], int t, const Q4<S>& v) {
+  b[0][t + 1] = v.r;
+  b[1][t + 1] = v.u;
+  b[2][t + 1] = v.v;
+  b[3][t + 1] = v.p;
+  if (t == 0 || t == 64 * W - 1) {
+    const int e = t == 0 ? 0 : 64 * W + 1;
+    b[0][e] = v.r;
+    b[1][e] = v.u;
+    b[2][e] = v.v;
+    b[3][e] = v.p;
+  }
+}
+template <typename S, int W>
+__device__ __forceinline__ Q4<S> lds_get(const S (&b)[4][64 * W + 2], int slot) {
+  return Q4<S>{b[0][slot], b[1][slot], b[2][slot], b[3][slot]};
+}
+
+// The 64*W threads of a workgroup own 64*W consecutive cells of row jrow
+// (padded columns (64*W - 4)*seg + t) and update the middle 64*W - 4: the
+// same arithmetic as x_segment, with the neighbour values passed through LDS
+// (no VALU lane moves) and the 4-cell halo paid once per workgroup instead of
+// once per wave.  Every thread of the workgroup calls it with the same
+// (jrow, seg): it has barriers.  Consecutive calls may reuse `sh`: each
+// buffer is rewritten only after a barrier that follows its last read.
+template <typename S, bool ST1, bool COUR, int W>
+__device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
+                                        const TraceK<S>& tk, double dtdx,
+                                        const GridView<S>& in, const GridView<S>& out,
+                                        long jrow, int seg, XLds<S, W>& sh, double& cx,
+                                        double& cy) {
+  constexpr int N = 64 * W;
+  const int t = threadIdx.x;
+  const long j = jrow + 2;
+  const int i = (N - 4) * seg + t;  // padded column of this thread
+  const int ic = i < g.nx + 4 ? i : g.nx + 3;
+  const long o = j * g.pitch + ic;
+
+  const S U0 = in.p[0][o], U1 = in.p[1][o], U2 = in.p[2][o], U3 = in.p[3][o];
+  const Prim<S> pc = prim_cell<S>(U0, U1, U2, U3, ph);
+  Q4<S> dq = {(S) 0, (S) 0, (S) 0, (S) 0};
+  if (ph.iorder != 1) {  // launch-uniform
+    lds_put<S, W>(sh.q, t, pc.q);
+    __syncthreads();
+    dq = slope4<S, ST1>(lds_get<S, W>(sh.q, t), pc.q, lds_get<S, W>(sh.q, t + 2),
+                        ph.slope_type);
+  }
+  Q4<S> qxm, qxp;
+  trace_cell<S>(pc.q, dq, pc.c, tk, qxm, qxp);
+  lds_put<S, W>(sh.x, t, qxp);
+  __syncthreads();
+  const Q4<S> qr = lds_get<S, W>(sh.x, t + 2);
+  // flux through the face between this thread's cell and the next one
+  const Q4<S> fr = riemann_flux<S>(qxm, qr, ph);
+  lds_put<S, W>(sh.f, t, fr);
+  __syncthreads();
+  const Q4<S> fl = lds_get<S, W>(sh.f, t);
+
+  const bool wr = t >= 2 && t <= N - 3 && i <= g.nx + 1;
+  const double n0 = update1<S>(U0, fl.r, fr.r, dtdx);
+  const double n1 = update1<S>(U1, fl.u, fr.u, dtdx);
+  const double n2 = update1<S>(U2, fl.v, fr.v, dtdx);
+  const double n3 = update1<S>(U3, fl.p, fr.p, dtdx);
+  if (wr) {
+    out.p[0][o] = (S) n0;
+    out.p[1][o] = (S) n1;
+    out.p[2][o] = (S) n2;
+    out.p[3][o] = (S) n3;
+    if (COUR) {
+      S ax, ay;
+      courant_cell<S>((S) n0, (S) n1, (S) n2, (S) n3, ph, ax, ay);
+      cx = maxm(cx, (double) ax);
+      cy = maxm(cy, (double) ay);
+    }
+  }
+}
+
 // ---- y sweep: one column, rows [j0, j1) ----------------------------------------
 
 template <typename S>

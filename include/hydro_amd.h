@@ -89,7 +89,10 @@ int hydro_amd_grid_set_state(hydro_amd_grid *g, double t, int nstep, double dt);
 int hydro_amd_grid_set_limits(hydro_amd_grid *g, double tend, int nstepmax);
 /* Sweep launch shape (results do not depend on it): rows each y-sweep lane
  * marches, waves per y-sweep workgroup (1..4), waves per x-sweep workgroup
- * (1..4) and row segments each x-sweep wave walks. */
+ * (1..4) and row pieces each x-sweep workgroup walks.  With 2 or 4 x waves
+ * the workgroup shares one row piece of 64*xwaves cells through LDS; with 1
+ * or 3 each wave takes its own 64-cell piece and exchanges neighbours by DPP
+ * lane rotations. */
 int hydro_amd_grid_set_tune(hydro_amd_grid *g, int yseg, int ywaves, int xwaves, int xiter);
 /* Autotuner: times the candidate launch shapes of both sweeps on this grid
  * (reps launches each, HIP events; the sweeps read the state and write only
