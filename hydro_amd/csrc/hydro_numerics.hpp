@@ -113,6 +113,21 @@ __device__ __forceinline__ void sqrt_fast2(double x, double y, double& sx, doubl
 // correctly rounded multi-instruction sequences.
 __device__ __forceinline__ double xdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float xdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+// c + a/b, c - a/b and a/b - c: in FP64 exactly as written (the quotient
+// rounded, then the sum); in FP32 the quotient a * rcp(b) fuses into the sum
+// (one v_fma_f32).
+__device__ __forceinline__ double add_div(double c, double a, double b) { return c + a / b; }
+__device__ __forceinline__ float add_div(float c, float a, float b) {
+  return __builtin_fmaf(a, __builtin_amdgcn_rcpf(b), c);
+}
+__device__ __forceinline__ double sub_div(double c, double a, double b) { return c - a / b; }
+__device__ __forceinline__ float sub_div(float c, float a, float b) {
+  return __builtin_fmaf(-a, __builtin_amdgcn_rcpf(b), c);
+}
+__device__ __forceinline__ double div_sub(double a, double b, double c) { return a / b - c; }
+__device__ __forceinline__ float div_sub(float a, float b, float c) {
+  return __builtin_fmaf(a, __builtin_amdgcn_rcpf(b), -c);
+}
 __device__ __forceinline__ double xsqrt(double a) { return sqrt_fast(a); }
 __device__ __forceinline__ float xsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 __device__ __forceinline__ void xsqrt2(double a, double b, double& sa, double& sb) {
@@ -169,7 +184,7 @@ __device__ __forceinline__ Prim<R> prim_cell(R U0, R U1, R U2, R U3, const Phys<
   R u = xdiv(U1, r);
   R v = xdiv(U2, r);
   R eken = half * (u * u + v * v);
-  R e = xdiv(U3, r) - eken;
+  R e = div_sub(U3, r, eken);
   R p = ph.gm1 * r * e;
   p = maxm(p, r * ph.smallp);
   o.c = xsqrt(xdiv(ph.gamma * p, r));
@@ -258,17 +273,16 @@ __device__ __forceinline__ void trace_cell(const Q4<R>& q, const Q4<R>& dq, R cc
   R dprcc = xdiv(dq.p, q.r * cc);
   R am = half * (dprcc - dq.u) * rOcc;
   R ap = half * (dprcc + dq.u) * rOcc;
-  R a0r = dq.r - xdiv(dq.p, csq);
+  R a0r = sub_div(dq.r, dq.p, csq);
   R a0v = dq.v;
   R upcc = q.u + cc;
   R umcc = q.u - cc;
-  R upccx = upcc * k.dtdx;
-  R umccx = umcc * k.dtdx;
-  R ux = q.u * k.dtdx;
   {
-    R spm = (umcc >= k.zeror) ? k.project : umccx + one;
-    R spp = (upcc >= k.zeror) ? k.project : upccx + one;
-    R sp0 = (q.u >= k.zeror) ? k.project : ux + one;
+    // (umcc * dtdx + one: one multiply-add in FP32, the reference's
+    // product then sum in FP64)
+    R spm = (umcc >= k.zeror) ? k.project : umcc * k.dtdx + one;
+    R spp = (upcc >= k.zeror) ? k.project : upcc * k.dtdx + one;
+    R sp0 = (q.u >= k.zeror) ? k.project : q.u * k.dtdx + one;
     R apr = -half * spp * ap;
     R amr = -half * spm * am;
     R azr = -half * sp0 * a0r;
@@ -279,9 +293,9 @@ __device__ __forceinline__ void trace_cell(const Q4<R>& q, const Q4<R>& dq, R cc
     qxp.p = q.p + (apr + amr) * csq;
   }
   {
-    R spm = (umcc <= k.zerol) ? -k.project : umccx - one;
-    R spp = (upcc <= k.zerol) ? -k.project : upccx - one;
-    R sp0 = (q.u <= k.zerol) ? -k.project : ux - one;
+    R spm = (umcc <= k.zerol) ? -k.project : umcc * k.dtdx - one;
+    R spp = (upcc <= k.zerol) ? -k.project : upcc * k.dtdx - one;
+    R sp0 = (q.u <= k.zerol) ? -k.project : q.u * k.dtdx - one;
     R apl = -half * spp * ap;
     R aml = -half * spm * am;
     R azl = -half * sp0 * a0r;
@@ -335,16 +349,17 @@ __device__ __forceinline__ Q4<R> riemann_flux(const Q4<R>& ql, const Q4<R>& qr,
   // Star state and sampling at x/t = 0 (riemann.c:211-276).
   R ps = (R) pst;
   R wr, wl;
-  xsqrt2(cr * (one + xdiv((R) g6 * (ps - pr), pr)), cl * (one + xdiv((R) g6 * (ps - pl), pl)),
+  xsqrt2(cr * add_div(one, (R) g6 * (ps - pr), pr), cl * add_div(one, (R) g6 * (ps - pl), pl),
          wr, wl);
-  R ustar = half * (ul + xdiv(pl - ps, wl) + ur - xdiv(pr - ps, wr));
+  // ((ul + (pl-ps)/wl) + ur) - (pr-ps)/wr, the reference's left-to-right sum
+  R ustar = half * sub_div(add_div(ul, pl - ps, wl) + ur, pr - ps, wr);
   bool left = ustar > (R) 0;
   R sg = left ? (R) 1 : (R) -1;
   R ro = left ? rl : rr;
   R uo = left ? ul : ur;
   R po = left ? pl : pr;
   R wo = left ? wl : wr;
-  R rstar = xdiv(ro, one + xdiv(ro * (po - ps), wo * wo));
+  R rstar = xdiv(ro, add_div(one, ro * (po - ps), wo * wo));
   rstar = fmax(rstar, ph.smallr);
   R co, cstar;  // (riemann.c computes co before rstar; neither depends on the other)
   xsqrt2(fabs(xdiv(ph.gamma * po, ro)), fabs(xdiv(ph.gamma * ps, rstar)), co, cstar);
@@ -358,7 +373,7 @@ __device__ __forceinline__ Q4<R> riemann_flux(const Q4<R>& ql, const Q4<R>& qr,
     spout = ushock;
   }
   R scr = fmax(spout - spin, ph.smallc + fabs(spout + spin));
-  R frac = (one + xdiv(spout + spin, scr)) * half;
+  R frac = add_div(one, spout + spin, scr) * half;
   frac = fmax((R) 0, fmin(one, frac));
   R gr, gu, gp;
   if (spout < (R) 0) {
