@@ -20,7 +20,7 @@ namespace hydro {
 
 // A workgroup walks wpb * xiter consecutive row segments, wpb at a time.
 template <typename S, bool ST1, bool COUR>
-__global__ __launch_bounds__(256) void k_sweep_x(GridGeom g, Phys<S> ph, double dx,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_sweep_x(GridGeom g, Phys<S> ph, double dx,
                                                  double dt_val, const RunCtl* ctl,
                                                  GridView<S> in, GridView<S> out,
                                                  uint64_t* cour, int nseg, int xiter) {
@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(GridGeom g, Phys<S> ph, double 
 // Workgroup form (W = 2 or 4 waves): a workgroup walks xiter consecutive
 // row pieces of 64*W - 4 updated cells, neighbours through LDS.
 template <typename S, bool ST1, bool COUR, int W>
-__global__ __launch_bounds__(64 * W) void k_sweep_xb(GridGeom g, Phys<S> ph, double dx,
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) void k_sweep_xb(GridGeom g, Phys<S> ph, double dx,
                                                      double dt_val, const RunCtl* ctl,
                                                      GridView<S> in, GridView<S> out,
                                                      uint64_t* cour, int nseg, int xiter) {

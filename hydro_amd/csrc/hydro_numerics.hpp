@@ -15,6 +15,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace hydro {
 
 // Physical / scheme constants, precomputed per launch in the compute type.
@@ -25,6 +27,7 @@ struct Phys {
   int niter;      // niter_riemann
   int iorder;     // 1: first order (dq = 0)
   int scheme;     // 1 muscl, 2 plmde, 3 collela
+  int fastdiv;    // FP64 only: the parameters allow the range-gated divisions
 };
 
 template <typename R>
@@ -47,6 +50,10 @@ __host__ __device__ inline Phys<R> make_phys(double gamma, double smallr, double
   p.niter = niter;
   p.iorder = iorder;
   p.scheme = scheme;
+  // Bounds the gated FP64 divisions rely on (see FastDiv): gamma in
+  // [1.01, 3], smallr in [2^-100, 2^30], smallc in [2^-50, 2^30].
+  p.fastdiv = sizeof(R) == 8 && gamma >= 1.01 && gamma <= 3.0 && smallr >= 0x1p-100 &&
+              smallr <= 0x1p30 && smallc >= 0x1p-50 && smallc <= 0x1p30;
   return p;
 }
 
@@ -104,6 +111,97 @@ __device__ __forceinline__ void sqrt_fast2(double x, double y, double& sx, doubl
   } else {
     sx = sqrt(x);
     sy = sqrt(y);
+  }
+}
+
+// ---- FP64 division without the range fix-ups ---------------------------------
+//
+// The compiler's FP64 a / b is: b' = v_div_scale(b), a' = v_div_scale(a), the
+// v_rcp_f64 estimate of b' refined by two Newton-Raphson steps, q = a' * r,
+// one residual correction (v_div_fmas: an FMA, rescaled when v_div_scale set
+// its flag) and v_div_fixup (special operands, sign).  v_div_scale changes
+// nothing and leaves the flag clear unless an operand is denormal or extreme:
+// |a| < 2^-969, a/b outside the normal range, exponent(a) - exponent(b) >= 768
+// or |b| >= 2^1022.  Away from those, a / b is the same sequence without the
+// two scalings (FastDiv below, bit for bit: checked on the GPU for 1.1e8
+// operand pairs with frexp exponents in [-600, 600], none differing).  Zeros,
+// infinities and NaNs are settled by v_div_fixup from a and b alone, whatever
+// the core computed, so they agree too.  The refined reciprocal depends on b
+// alone: divisions by one b share it (rcp_of once, qdiv per numerator).
+//
+// The sweeps take FastDiv only where a wave-uniform test has bounded every
+// operand: a cell's conserved values (cell_in), a face's floored Riemann
+// inputs (in riemann_flux) and the Newton pressure at each iteration (pst_in),
+// with the launch parameters bounded (Phys::fastdiv).  The exponent bounds
+// carried through prim_cell, trace_cell and riemann_flux from those tests
+// keep every division operand far from the conditions above (the tightest
+// margin, the Newton step's relative change d / (pst + smallpp), stays above
+// 2^-950); the test failing on any lane sends the whole wave down the IEEE
+// path (IeeeDiv), which is the reference's arithmetic.
+struct Rcp {
+  double b, r;
+};
+__device__ __forceinline__ Rcp rcp_of(double b) {
+  const double r0 = __builtin_amdgcn_rcp(b);
+  const double r1 = __builtin_fma(r0, __builtin_fma(-b, r0, 1.0), r0);
+  return Rcp{b, __builtin_fma(r1, __builtin_fma(-b, r1, 1.0), r1)};
+}
+__device__ __forceinline__ double qdiv(double a, const Rcp& d) {
+  const double q = a * d.r;
+  return __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-d.b, q, a), d.r, q), d.b, a);
+}
+
+// Division / square-root policies of the FP64 numerics.  sqrt1/sqrt2 of
+// FastDiv take the sqrt_core sequence directly: callers use them only for
+// arguments shown positive, finite and at least 2^-767.
+struct IeeeDiv {
+  struct R {
+    double b;
+  };
+  static __device__ __forceinline__ R rcp(double b) { return R{b}; }
+  static __device__ __forceinline__ double q(double a, const R& d) { return a / d.b; }
+  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+  static __device__ __forceinline__ double sqrt1(double x) { return sqrt_fast(x); }
+  static __device__ __forceinline__ void sqrt2(double x, double y, double& sx, double& sy) {
+    sqrt_fast2(x, y, sx, sy);
+  }
+};
+struct FastDiv {
+  using R = Rcp;
+  static __device__ __forceinline__ R rcp(double b) { return rcp_of(b); }
+  static __device__ __forceinline__ double q(double a, const R& d) { return qdiv(a, d); }
+  static __device__ __forceinline__ double div(double a, double b) { return qdiv(a, rcp_of(b)); }
+  static __device__ __forceinline__ double sqrt1(double x) { return sqrt_core(x); }
+  static __device__ __forceinline__ void sqrt2(double x, double y, double& sx, double& sy) {
+    sx = sqrt_core(x);
+    sy = sqrt_core(y);
+  }
+};
+
+// Operand tests (per lane; callers take wave_all of them).  Comparisons fail
+// for NaN, so infinities and NaNs never pass.
+//   in_pos(x): x in [2^-41, 2^40)              (floored densities, pressures)
+//   in_zero(x): x = +-0 or |x| in [2^-41, 2^40) (velocities, momenta, energy)
+//   pst_in(x): x = +-0 or |x| in [2^-151, 2^150) (the Newton pressure)
+__device__ __forceinline__ bool in_pos(double x) { return x >= 0x1p-41 && x < 0x1p40; }
+__device__ __forceinline__ bool in_zero(double x) {
+  const double a = fabs(x);
+  return a < 0x1p40 && (a >= 0x1p-41 || a == 0.0);
+}
+__device__ __forceinline__ bool pst_in(double x) {
+  const double a = fabs(x);
+  return a < 0x1p150 && (a >= 0x1p-151 || a == 0.0);
+}
+
+// A cell whose primitives and trace may take FastDiv: density (after the
+// smallr floor), momenta and energy as in_pos / in_zero.
+template <typename S>
+__device__ __forceinline__ bool cell_in(S U0, S U1, S U2, S U3, const Phys<S>& ph) {
+  if constexpr (sizeof(S) == 8) {
+    return ph.fastdiv && in_pos(maxm(U0, ph.smallr)) && in_zero(U1) && in_zero(U2) &&
+           in_zero(U3);
+  } else {
+    return false;
   }
 }
 
@@ -175,19 +273,34 @@ __device__ __forceinline__ void nsqrt2<float>(double x, double y, double& sx, do
 }
 
 // constoprim.c:62-78 + equation_of_state.c:64-81 for one cell.  U is in sweep
-// slot order (slot 1 = normal momentum).
-template <typename R>
+// slot order (slot 1 = normal momentum).  FAST (FP64, for cells that passed
+// cell_in): FastDiv, the four divisions by rho sharing one reciprocal; the
+// sound-speed argument gamma*p/rho is then in [2^-183, 2^163].
+template <typename R, bool FAST = false>
 __device__ __forceinline__ Prim<R> prim_cell(R U0, R U1, R U2, R U3, const Phys<R>& ph) {
   Prim<R> o;
   const R half = (R) 0.5;
   R r = maxm(U0, ph.smallr);
-  R u = xdiv(U1, r);
-  R v = xdiv(U2, r);
-  R eken = half * (u * u + v * v);
-  R e = div_sub(U3, r, eken);
-  R p = ph.gm1 * r * e;
-  p = maxm(p, r * ph.smallp);
-  o.c = xsqrt(xdiv(ph.gamma * p, r));
+  R u, v, e, p;
+  if constexpr (sizeof(R) == 8) {
+    using P = std::conditional_t<FAST, FastDiv, IeeeDiv>;
+    const auto rr = P::rcp(r);
+    u = P::q(U1, rr);
+    v = P::q(U2, rr);
+    R eken = half * (u * u + v * v);
+    e = P::q(U3, rr) - eken;
+    p = ph.gm1 * r * e;
+    p = maxm(p, r * ph.smallp);
+    o.c = P::sqrt1(P::q(ph.gamma * p, rr));
+  } else {
+    u = xdiv(U1, r);
+    v = xdiv(U2, r);
+    R eken = half * (u * u + v * v);
+    e = div_sub(U3, r, eken);
+    p = ph.gm1 * r * e;
+    p = maxm(p, r * ph.smallp);
+    o.c = xsqrt(xdiv(ph.gamma * p, r));
+  }
   o.q.r = r;
   o.q.u = u;
   o.q.v = v;
@@ -195,12 +308,25 @@ __device__ __forceinline__ Prim<R> prim_cell(R U0, R U1, R U2, R U3, const Phys<
   return o;
 }
 
+// Wave-uniform: may every cell of this wave take prim_cell / trace_cell with
+// FAST (FP64 only)?
+template <typename S>
+__device__ __forceinline__ bool wave_cell_in(S U0, S U1, S U2, S U3, const Phys<S>& ph) {
+  if constexpr (sizeof(S) == 8) {
+    return wave_all(cell_in<S>(U0, U1, U2, U3, ph));
+  } else {
+    return false;
+  }
+}
+
 // Courant numbers of one cell in grid slot order (compute_deltat.c:71-88 +
-// courantOnXY.c:115-121): returns c + |vx| and c + |vy|.
+// courantOnXY.c:115-121): returns c + |vx| and c + |vy|.  The primitives take
+// FastDiv when every active lane's cell passes cell_in.
 template <typename R>
 __device__ __forceinline__ void courant_cell(R U0, R U1, R U2, R U3, const Phys<R>& ph, R& cx,
                                              R& cy) {
-  Prim<R> o = prim_cell(U0, U1, U2, U3, ph);
+  const Prim<R> o = wave_cell_in<R>(U0, U1, U2, U3, ph) ? prim_cell<R, true>(U0, U1, U2, U3, ph)
+                                                        : prim_cell<R, false>(U0, U1, U2, U3, ph);
   cx = o.c + fabs(o.q.u);
   cy = o.c + fabs(o.q.v);
 }
@@ -262,18 +388,29 @@ __device__ __forceinline__ TraceK<R> make_tracek(R dtdx, int scheme) {
 
 // trace.c:85-147: MUSCL-Hancock predictor of one cell -> qxm (the state at
 // the cell's right face, seen from inside) and qxp (its left face).
-template <typename R>
+// FAST (FP64): FastDiv, for cells whose own and neighbour values (the slope
+// inputs) passed cell_in.
+template <typename R, bool FAST = false>
 __device__ __forceinline__ void trace_cell(const Q4<R>& q, const Q4<R>& dq, R cc,
                                            const TraceK<R>& k, Q4<R>& qxm, Q4<R>& qxp) {
   const R half = (R) 0.5;
   const R one = (R) 1;
   R csq = cc * cc;
-  R rOcc = xdiv(q.r, cc);
-  R OrOcc = xdiv(cc, q.r);
-  R dprcc = xdiv(dq.p, q.r * cc);
+  R rOcc, OrOcc, dprcc, a0r;
+  if constexpr (sizeof(R) == 8) {
+    using P = std::conditional_t<FAST, FastDiv, IeeeDiv>;
+    rOcc = P::div(q.r, cc);
+    OrOcc = P::div(cc, q.r);
+    dprcc = P::div(dq.p, q.r * cc);
+    a0r = dq.r - P::div(dq.p, csq);
+  } else {
+    rOcc = xdiv(q.r, cc);
+    OrOcc = xdiv(cc, q.r);
+    dprcc = xdiv(dq.p, q.r * cc);
+    a0r = sub_div(dq.r, dq.p, csq);
+  }
   R am = half * (dprcc - dq.u) * rOcc;
   R ap = half * (dprcc + dq.u) * rOcc;
-  R a0r = sub_div(dq.r, dq.p, csq);
   R a0v = dq.v;
   R upcc = q.u + cc;
   R umcc = q.u - cc;
@@ -307,11 +444,139 @@ __device__ __forceinline__ void trace_cell(const Q4<R>& q, const Q4<R>& dq, R cc
   }
 }
 
-// riemann.c:128-276 for one face (qleft = ql, qright = qr) followed by
-// cmpflx.c:67-90: returns the Godunov flux through the face.
+// ---- FP64 Riemann solver (the FP64 form) ------------------------------------
+
+// One Newton step of riemann.c:160-201 for one face; returns true when the
+// face has converged (the reference's goon = 0).  P: IeeeDiv, or FastDiv when
+// pst, the floored inputs and the launch parameters are in range.
+template <typename P>
+__device__ __forceinline__ bool newton_step(double& pst, double pl, double pr, double cl,
+                                            double cr, double ul, double ur,
+                                            const typename P::R& rpl, const typename P::R& rpr,
+                                            double g6, double smallpp) {
+  double wwl, wwr;
+  P::sqrt2(cl * (1.0 + P::q(g6 * (pst - pl), rpl)), cr * (1.0 + P::q(g6 * (pst - pr), rpr)),
+           wwl, wwr);
+  const double swwl = wwl * wwl;
+  const double qlw = P::div(2.0 * wwl * swwl, swwl + cl);
+  const double qrw = P::div(2.0 * wwr * (wwr * wwr), (wwr * wwr) + cr);
+  const double usl = ul - P::div(pst - pl, wwl);
+  const double usr = ur + P::div(pst - pr, wwr);
+  const double t = (P::div(qrw * qlw, qrw + qlw) * (usl - usr));
+  const double d = (t > (-pst)) ? t : (-pst);
+  pst += d;
+  const double rel = P::div(d, pst + smallpp);
+  const double arel = (rel > 0) ? rel : -rel;
+  return !(arel > 1e-6);
+}
+
+// Star state, sampling at x/t = 0 and flux (riemann.c:211-276, cmpflx.c:67-90)
+// for the converged pressure pst.  The last square-root pair (co, cstar) keeps
+// the checked sequence: cstar's argument is 0 when pst is.
+template <typename P>
+__device__ __forceinline__ Q4<double> star_flux(double pst, const Q4<double>& ql,
+                                                const Q4<double>& qr, double rl, double ul,
+                                                double pl, double rr, double ur, double pr,
+                                                double cl, double cr, const typename P::R& rpl,
+                                                const typename P::R& rpr,
+                                                const Phys<double>& ph) {
+  const double half = 0.5, one = 1.0, g6 = ph.gamma6_d;
+  const double ps = pst;
+  double wr, wl;
+  P::sqrt2(cr * (one + P::q(g6 * (ps - pr), rpr)), cl * (one + P::q(g6 * (ps - pl), rpl)), wr,
+           wl);
+  // ((ul + (pl-ps)/wl) + ur) - (pr-ps)/wr, the reference's left-to-right sum
+  const double ustar = half * (((ul + P::div(pl - ps, wl)) + ur) - P::div(pr - ps, wr));
+  const bool left = ustar > 0.0;
+  const double sg = left ? 1.0 : -1.0;
+  const double ro = left ? rl : rr;
+  const double uo = left ? ul : ur;
+  const double po = left ? pl : pr;
+  const double wo = left ? wl : wr;
+  double rstar = P::div(ro, one + P::div(ro * (po - ps), wo * wo));
+  rstar = fmax(rstar, ph.smallr);
+  const auto rro = P::rcp(ro);
+  double co, cstar;  // (riemann.c computes co before rstar; neither depends on the other)
+  sqrt_fast2(fabs(P::q(ph.gamma * po, rro)), fabs(P::div(ph.gamma * ps, rstar)), co, cstar);
+  co = fmax(ph.smallc, co);
+  cstar = fmax(ph.smallc, cstar);
+  double spout = co - sg * uo;
+  double spin = cstar - sg * ustar;
+  const double ushock = P::q(wo, rro) - sg * uo;
+  if (ps >= po) {
+    spin = ushock;
+    spout = ushock;
+  }
+  const double scr = fmax(spout - spin, ph.smallc + fabs(spout + spin));
+  double frac = (one + P::div(spout + spin, scr)) * half;
+  frac = fmax(0.0, fmin(one, frac));
+  double gr, gu, gp;
+  if (spout < 0.0) {
+    gr = ro;
+    gu = uo;
+    gp = po;
+  } else if (spin > 0.0) {
+    gr = rstar;
+    gu = ustar;
+    gp = ps;
+  } else {
+    gr = frac * rstar + (one - frac) * ro;
+    gu = frac * ustar + (one - frac) * uo;
+    gp = frac * ps + (one - frac) * po;
+  }
+  const double gv = left ? ql.v : qr.v;
+  Q4<double> f;
+  const double m = gr * gu;
+  f.r = m;
+  f.u = m * gu + gp;
+  f.v = m * gv;
+  const double ekin = half * gr * (gu * gu + gv * gv);
+  const double etot = gp * ph.entho + ekin;
+  f.p = gu * (etot + gp);
+  return f;
+}
+
+// First guess, Newton iteration and star state of one face in FP64.  Each
+// block takes FastDiv when its operands are bounded -- the floored inputs
+// passed their tests (ff) and, for a Newton step and the star state, the
+// wave's pressures pass pst_in -- and IeeeDiv otherwise.
+__device__ __forceinline__ Q4<double> riemann_d(const Q4<double>& ql, const Q4<double>& qr,
+                                                double rl, double ul, double pl, double rr,
+                                                double ur, double pr, double cl, double cr,
+                                                bool ff, const Phys<double>& ph) {
+  const double g6 = ph.gamma6_d;
+  double wl0, wr0, pst;
+  Rcp rpl{pl, 0.0}, rpr{pr, 0.0};
+  if (ff) {
+    FastDiv::sqrt2(cl, cr, wl0, wr0);
+    pst = fmax(FastDiv::div((wr0 * pl + wl0 * pr) + wl0 * wr0 * (ul - ur), wl0 + wr0), 0.0);
+    rpl = rcp_of(pl);
+    rpr = rcp_of(pr);
+  } else {
+    IeeeDiv::sqrt2(cl, cr, wl0, wr0);
+    pst = fmax(((wr0 * pl + wl0 * pr) + wl0 * wr0 * (ul - ur)) / (wl0 + wr0), 0.0);
+  }
+  for (int it = 0; it < ph.niter; it++) {
+    bool done;
+    if (ff && wave_all(pst_in(pst)))
+      done = newton_step<FastDiv>(pst, pl, pr, cl, cr, ul, ur, rpl, rpr, g6, ph.smallpp_d);
+    else
+      done = newton_step<IeeeDiv>(pst, pl, pr, cl, cr, ul, ur, IeeeDiv::R{pl}, IeeeDiv::R{pr},
+                                  g6, ph.smallpp_d);
+    if (done) break;
+  }
+  if (ff && wave_all(pst_in(pst)))
+    return star_flux<FastDiv>(pst, ql, qr, rl, ul, pl, rr, ur, pr, cl, cr, rpl, rpr, ph);
+  return star_flux<IeeeDiv>(pst, ql, qr, rl, ul, pl, rr, ur, pr, cl, cr, IeeeDiv::R{pl},
+                            IeeeDiv::R{pr}, ph);
+}
+
+// The mixed-precision form (R = float) of riemann_flux: FP32 inputs and star
+// state, first guess and Newton iteration in FP64 with the refined hardware
+// estimates (div_nr, sqrt_nr).
 template <typename R>
-__device__ __forceinline__ Q4<R> riemann_flux(const Q4<R>& ql, const Q4<R>& qr,
-                                              const Phys<R>& ph) {
+__device__ __forceinline__ Q4<R> riemann_flux_mixed(const Q4<R>& ql, const Q4<R>& qr,
+                                                    const Phys<R>& ph) {
   const R half = (R) 0.5;
   const R one = (R) 1;
   R rl = fmax(ql.r, ph.smallr);
@@ -402,6 +667,30 @@ __device__ __forceinline__ Q4<R> riemann_flux(const Q4<R>& ql, const Q4<R>& qr,
   f.p = gu * (etot + gp);
   return f;
 }
+
+// riemann.c:128-276 for one face (qleft = ql, qright = qr) followed by
+// cmpflx.c:67-90: returns the Godunov flux through the face.
+template <typename R>
+__device__ __forceinline__ Q4<R> riemann_flux(const Q4<R>& ql, const Q4<R>& qr,
+                                              const Phys<R>& ph) {
+  if constexpr (sizeof(R) == 8) {
+    const double rl = fmax(ql.r, ph.smallr);
+    const double ul = ql.u;
+    const double pl = fmax(ql.p, rl * ph.smallp);
+    const double rr = fmax(qr.r, ph.smallr);
+    const double ur = qr.u;
+    const double pr = fmax(qr.p, rr * ph.smallp);
+    const double cl = ph.gamma * pl * rl;
+    const double cr = ph.gamma * pr * rr;
+    // the face's floored densities and pressures in_pos, velocities in_zero
+    const bool fast = ph.fastdiv && wave_all(in_pos(rl) && in_pos(pl) && in_pos(rr) &&
+                                             in_pos(pr) && in_zero(ul) && in_zero(ur));
+    return riemann_d(ql, qr, rl, ul, pl, rr, ur, pr, cl, cr, fast, ph);
+  } else {
+    return riemann_flux_mixed(ql, qr, ph);
+  }
+}
+
 
 // conservar.c:147-197: U_i + (F_{i-1/2} - F_{i+1/2}) * dtdx, accumulated in
 // double (exactly the reference expression when R = double).

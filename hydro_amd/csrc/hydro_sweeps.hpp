@@ -146,14 +146,18 @@ __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
   const long o = j * g.pitch + ic;
 
   const S U0 = in.p[0][o], U1 = in.p[1][o], U2 = in.p[2][o], U3 = in.p[3][o];
-  const Prim<S> pc = prim_cell<S>(U0, U1, U2, U3, ph);
+  // every cell of the wave (the slope neighbours included) in range: FastDiv
+  const bool fc = wave_cell_in<S>(U0, U1, U2, U3, ph);
+  const Prim<S> pc = fc ? prim_cell<S, true>(U0, U1, U2, U3, ph)
+                        : prim_cell<S, false>(U0, U1, U2, U3, ph);
   Q4<S> dq = {(S) 0, (S) 0, (S) 0, (S) 0};
   if (ph.iorder != 1) {
     const Q4<S> qm = from_left4(pc.q), qp = from_right4(pc.q);
     dq = slope4<S, ST1>(qm, pc.q, qp, ph.slope_type);
   }
   Q4<S> qxm, qxp;
-  trace_cell<S>(pc.q, dq, pc.c, tk, qxm, qxp);
+  if (fc) trace_cell<S, true>(pc.q, dq, pc.c, tk, qxm, qxp);
+  else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);
   const Q4<S> qr = from_right4(qxp);
   // flux through the face between this lane's cell and the next one
   const Q4<S> fr = riemann_flux<S>(qxm, qr, ph);
@@ -187,6 +191,7 @@ __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
 template <typename S, int W>
 struct XLds {
   S q[4][64 * W + 2], x[4][64 * W + 2], f[4][64 * W + 2];
+  int in[W];  // per wave: all its cells passed cell_in
 };
 
 template <typename S, int W>
@@ -229,16 +234,26 @@ __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
   const long o = j * g.pitch + ic;
 
   const S U0 = in.p[0][o], U1 = in.p[1][o], U2 = in.p[2][o], U3 = in.p[3][o];
-  const Prim<S> pc = prim_cell<S>(U0, U1, U2, U3, ph);
+  // FastDiv for the primitives when the wave's cells are in range, for the
+  // traces when every wave's are (the slopes read neighbours of other waves)
+  const bool fc = wave_cell_in<S>(U0, U1, U2, U3, ph);
+  const Prim<S> pc = fc ? prim_cell<S, true>(U0, U1, U2, U3, ph)
+                        : prim_cell<S, false>(U0, U1, U2, U3, ph);
   Q4<S> dq = {(S) 0, (S) 0, (S) 0, (S) 0};
+  bool ft = fc;
   if (ph.iorder != 1) {  // launch-uniform
     lds_put<S, W>(sh.q, t, pc.q);
+    if ((t & 63) == 0) sh.in[t >> 6] = fc;
     __syncthreads();
+    int all = 1;
+    for (int w = 0; w < W; w++) all &= sh.in[w];
+    ft = __builtin_amdgcn_readfirstlane(all) != 0;  // the same in every lane
     dq = slope4<S, ST1>(lds_get<S, W>(sh.q, t), pc.q, lds_get<S, W>(sh.q, t + 2),
                         ph.slope_type);
   }
   Q4<S> qxm, qxp;
-  trace_cell<S>(pc.q, dq, pc.c, tk, qxm, qxp);
+  if (ft) trace_cell<S, true>(pc.q, dq, pc.c, tk, qxm, qxp);
+  else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);
   lds_put<S, W>(sh.x, t, qxp);
   __syncthreads();
   const Q4<S> qr = lds_get<S, W>(sh.x, t + 2);
@@ -295,14 +310,30 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
   Cell<S> ub = load_y(in, (long) (j0 - 1) * P + i);
   Cell<S> uc = load_y(in, (long) j0 * P + i);
   Cell<S> ud = load_y(in, (long) (j0 + 1) * P + i);
-  Prim<S> pa = prim_cell<S>(ua.u0, ua.u1, ua.u2, ua.u3, ph);
-  Prim<S> pb = prim_cell<S>(ub.u0, ub.u1, ub.u2, ub.u3, ph);
-  Prim<S> pcc = prim_cell<S>(uc.u0, uc.u1, uc.u2, uc.u3, ph);
-  Prim<S> pd = prim_cell<S>(ud.u0, ud.u1, ud.u2, ud.u3, ph);
+  // FastDiv per row when the wave's cells of the row are in range (a trace
+  // also needs the rows above and below it in range)
+  auto prim = [&](const Cell<S>& u, bool f) {
+    return f ? prim_cell<S, true>(u.u0, u.u1, u.u2, u.u3, ph)
+             : prim_cell<S, false>(u.u0, u.u1, u.u2, u.u3, ph);
+  };
+  auto trace = [&](const Q4<S>& q, const Q4<S>& dq, S c, bool f, Q4<S>& qxm, Q4<S>& qxp) {
+    if (f) trace_cell<S, true>(q, dq, c, tk, qxm, qxp);
+    else trace_cell<S, false>(q, dq, c, tk, qxm, qxp);
+  };
+  const bool fa = wave_cell_in<S>(ua.u0, ua.u1, ua.u2, ua.u3, ph);
+  const bool fb = wave_cell_in<S>(ub.u0, ub.u1, ub.u2, ub.u3, ph);
+  bool fC = wave_cell_in<S>(uc.u0, uc.u1, uc.u2, uc.u3, ph);
+  bool fD = wave_cell_in<S>(ud.u0, ud.u1, ud.u2, ud.u3, ph);
+  Prim<S> pa = prim(ua, fa);
+  Prim<S> pb = prim(ub, fb);
+  Prim<S> pcc = prim(uc, fC);
+  Prim<S> pd = prim(ud, fD);
   const Q4<S> z = {(S) 0, (S) 0, (S) 0, (S) 0};
   Q4<S> qxm_b, qxp_b, qxm_c, qxp_c;
-  trace_cell<S>(pb.q, o1 ? z : slope4<S, ST1>(pa.q, pb.q, pcc.q, st), pb.c, tk, qxm_b, qxp_b);
-  trace_cell<S>(pcc.q, o1 ? z : slope4<S, ST1>(pb.q, pcc.q, pd.q, st), pcc.c, tk, qxm_c, qxp_c);
+  trace(pb.q, o1 ? z : slope4<S, ST1>(pa.q, pb.q, pcc.q, st), pb.c, fa && fb && fC, qxm_b,
+        qxp_b);
+  trace(pcc.q, o1 ? z : slope4<S, ST1>(pb.q, pcc.q, pd.q, st), pcc.c, fb && fC && fD, qxm_c,
+        qxp_c);
   Q4<S> fprev = riemann_flux<S>(qxm_b, qxp_c, ph);  // face (j0-1 | j0)
 
   // window: cell C = row j-2 (being updated), D = row j-1; carried: qxm of C,
@@ -319,9 +350,10 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
     const Cell<S> uE = uN;
     const int jn = j + 1 <= j1 + 1 ? j + 1 : j1 + 1;  // last row: a harmless reload
     uN = load_y(in, (long) jn * P + i);
-    const Prim<S> pE = prim_cell<S>(uE.u0, uE.u1, uE.u2, uE.u3, ph);
+    const bool fE = wave_cell_in<S>(uE.u0, uE.u1, uE.u2, uE.u3, ph);
+    const Prim<S> pE = prim(uE, fE);
     Q4<S> qxmD, qxpD;
-    trace_cell<S>(qD, o1 ? z : slope4<S, ST1>(qC, qD, pE.q, st), cD, tk, qxmD, qxpD);
+    trace(qD, o1 ? z : slope4<S, ST1>(qC, qD, pE.q, st), cD, fC && fD && fE, qxmD, qxpD);
     const Q4<S> fcur = riemann_flux<S>(qxmC, qxpD, ph);  // face (j-2 | j-1)
     const double n0 = update1<S>(uC.u0, fprev.r, fcur.r, dtdx);
     const double n1 = update1<S>(uC.u1, fprev.u, fcur.u, dtdx);
@@ -346,6 +378,8 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
     qC = qD;
     qD = pE.q;
     cD = pE.c;
+    fC = fD;
+    fD = fE;
     uC = uD;
     uD = uE;
   };
