@@ -65,7 +65,8 @@ struct Q4 {  // rho, normal velocity, transverse velocity, pressure
 template <typename R>
 struct Prim {
   Q4<R> q;
-  R c;  // sound speed
+  R c;     // sound speed
+  R rinv;  // FP64 with FAST: the refined reciprocal of q.r (FastDiv)
 };
 
 template <typename R>
@@ -285,6 +286,7 @@ __device__ __forceinline__ Prim<R> prim_cell(R U0, R U1, R U2, R U3, const Phys<
   if constexpr (sizeof(R) == 8) {
     using P = std::conditional_t<FAST, FastDiv, IeeeDiv>;
     const auto rr = P::rcp(r);
+    if constexpr (FAST) o.rinv = rr.r;
     u = P::q(U1, rr);
     v = P::q(U2, rr);
     R eken = half * (u * u + v * v);
@@ -389,10 +391,12 @@ __device__ __forceinline__ TraceK<R> make_tracek(R dtdx, int scheme) {
 // trace.c:85-147: MUSCL-Hancock predictor of one cell -> qxm (the state at
 // the cell's right face, seen from inside) and qxp (its left face).
 // FAST (FP64): FastDiv, for cells whose own and neighbour values (the slope
-// inputs) passed cell_in.
-template <typename R, bool FAST = false>
+// inputs) passed cell_in.  RINV: rinv is prim_cell's reciprocal of q.r, reused
+// for cc / q.r.
+template <typename R, bool FAST = false, bool RINV = false>
 __device__ __forceinline__ void trace_cell(const Q4<R>& q, const Q4<R>& dq, R cc,
-                                           const TraceK<R>& k, Q4<R>& qxm, Q4<R>& qxp) {
+                                           const TraceK<R>& k, Q4<R>& qxm, Q4<R>& qxp,
+                                           R rinv = (R) 0) {
   const R half = (R) 0.5;
   const R one = (R) 1;
   R csq = cc * cc;
@@ -400,7 +404,8 @@ __device__ __forceinline__ void trace_cell(const Q4<R>& q, const Q4<R>& dq, R cc
   if constexpr (sizeof(R) == 8) {
     using P = std::conditional_t<FAST, FastDiv, IeeeDiv>;
     rOcc = P::div(q.r, cc);
-    OrOcc = P::div(cc, q.r);
+    if constexpr (FAST && RINV) OrOcc = qdiv(cc, Rcp{q.r, rinv});
+    else OrOcc = P::div(cc, q.r);
     dprcc = P::div(dq.p, q.r * cc);
     a0r = dq.r - P::div(dq.p, csq);
   } else {

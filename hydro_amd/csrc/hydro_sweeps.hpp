@@ -156,7 +156,7 @@ __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
     dq = slope4<S, ST1>(qm, pc.q, qp, ph.slope_type);
   }
   Q4<S> qxm, qxp;
-  if (fc) trace_cell<S, true>(pc.q, dq, pc.c, tk, qxm, qxp);
+  if (fc) trace_cell<S, true, true>(pc.q, dq, pc.c, tk, qxm, qxp, pc.rinv);
   else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);
   const Q4<S> qr = from_right4(qxp);
   // flux through the face between this lane's cell and the next one
@@ -252,7 +252,7 @@ __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
                         ph.slope_type);
   }
   Q4<S> qxm, qxp;
-  if (ft) trace_cell<S, true>(pc.q, dq, pc.c, tk, qxm, qxp);
+  if (ft) trace_cell<S, true, true>(pc.q, dq, pc.c, tk, qxm, qxp, pc.rinv);  // ft implies fc
   else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);
   lds_put<S, W>(sh.x, t, qxp);
   __syncthreads();
