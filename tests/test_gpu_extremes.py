@@ -2,11 +2,15 @@
 
 The sweeps take shortcuts that are exact only inside a range: the square
 root drops the library sequence's range fix-up when every lane's argument is
-in [2^-767, DBL_MAX], and the x sweep's limiter is specialised for
-slope_type == 1.  These states push arguments out of that range (tiny
-floors, near-vacuum cells, huge energies, zero and negative inputs) so the
-general paths run, and compare every value with the oracle (a NaN matches a
-NaN).
+in [2^-767, DBL_MAX]; divisions drop v_div_scale when a wave's cells and
+faces pass the range tests (densities and pressures in [2^-41, 2^40),
+velocities, momenta and energies zero or in that range, the Newton pressure
+within [2^-151, 2^150)); and the x sweep's limiter is specialised for
+slope_type == 1.  These states push arguments out of those ranges (tiny
+floors, near-vacuum cells, huge energies, zero and negative inputs, values
+straddling the division tests' bounds) so the general paths run, alone and
+beside the fast ones, and compare every value with the oracle (a NaN matches
+a NaN).
 """
 import numpy as np
 import pytest
@@ -87,3 +91,29 @@ def test_huge_and_zero_values():
     assert len(dts) >= 1
     assert same(dts, rdts), (dts, rdts)
     assert same(g, r)
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_division_gate_boundaries(seed):
+    """Densities, pressures and momenta scattered just inside and just outside
+    the division tests' bounds (2^-41, 2^40), in some columns and rows only:
+    waves and faces of one sweep take the fast and the IEEE divisions side by
+    side, and the result is the oracle's bit for bit."""
+    p = params(150, 90)
+    rng = np.random.default_rng(seed)
+    u = np.zeros(p.shape())
+    shape = (90, 150)
+    u[0, 2:-2, 2:-2] = 1.0 + 0.1 * rng.random(shape)
+    u[3, 2:-2, 2:-2] = 2.5 + 0.1 * rng.random(shape)
+    # columns 40..59 near the upper bound, rows 30..44 near the lower one
+    hi = 2.0 ** rng.uniform(39.5, 40.5, (90, 20))
+    u[0, 2:-2, 42:62] = hi
+    u[3, 2:-2, 42:62] = hi * rng.uniform(0.5, 2.0, (90, 20))
+    lo = 2.0 ** rng.uniform(-41.5, -40.5, (15, 150))
+    u[0, 32:47, 2:-2] = np.maximum(lo, 1e-9)
+    u[1, 32:47, 2:-2] = lo * rng.uniform(-1, 1, (15, 150))
+    u[3, 32:47, 2:-2] = lo * rng.uniform(0.5, 2.0, (15, 150))
+    u[3, 70:74, 100:104] = 1e6
+    g, dts, r, rdts = run_both(p, u, 6)
+    assert np.array_equal(dts, rdts), (dts, rdts)
+    assert same(g, r), f"{np.count_nonzero(~((g == r) | (np.isnan(g) & np.isnan(r))))} differ"
