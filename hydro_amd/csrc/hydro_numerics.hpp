@@ -280,6 +280,7 @@ __device__ __forceinline__ void nsqrt2<float>(double x, double y, double& sx, do
 template <typename R, bool FAST = false>
 __device__ __forceinline__ Prim<R> prim_cell(R U0, R U1, R U2, R U3, const Phys<R>& ph) {
   Prim<R> o;
+  o.rinv = (R) 0;
   const R half = (R) 0.5;
   R r = maxm(U0, ph.smallr);
   R u, v, e, p;
@@ -476,8 +477,8 @@ __device__ __forceinline__ bool newton_step(double& pst, double pl, double pr, d
 }
 
 // Star state, sampling at x/t = 0 and flux (riemann.c:211-276, cmpflx.c:67-90)
-// for the converged pressure pst.  The last square-root pair (co, cstar) keeps
-// the checked sequence: cstar's argument is 0 when pst is.
+// for the converged pressure pst.  With FastDiv the square roots take the
+// core alone (cstar's argument, 0 when pst is, returns +0 directly).
 template <typename P>
 __device__ __forceinline__ Q4<double> star_flux(double pst, const Q4<double>& ql,
                                                 const Q4<double>& qr, double rl, double ul,
@@ -502,7 +503,14 @@ __device__ __forceinline__ Q4<double> star_flux(double pst, const Q4<double>& ql
   rstar = fmax(rstar, ph.smallr);
   const auto rro = P::rcp(ro);
   double co, cstar;  // (riemann.c computes co before rstar; neither depends on the other)
-  sqrt_fast2(fabs(P::q(ph.gamma * po, rro)), fabs(P::div(ph.gamma * ps, rstar)), co, cstar);
+  if constexpr (std::is_same_v<P, FastDiv>) {
+    // both arguments bounded; cstar's is +0 when ps is (sqrt(+0) = +0)
+    const double xs = fabs(P::div(ph.gamma * ps, rstar));
+    co = sqrt_core(fabs(P::q(ph.gamma * po, rro)));
+    cstar = xs == 0.0 ? xs : sqrt_core(xs);
+  } else {
+    sqrt_fast2(fabs(P::q(ph.gamma * po, rro)), fabs(P::div(ph.gamma * ps, rstar)), co, cstar);
+  }
   co = fmax(ph.smallc, co);
   cstar = fmax(ph.smallc, cstar);
   double spout = co - sg * uo;

@@ -339,7 +339,7 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
   // window: cell C = row j-2 (being updated), D = row j-1; carried: qxm of C,
   // flux through the face below C
   Q4<S> qC = pcc.q, qD = pd.q;
-  S cD = pd.c;
+  S cD = pd.c, rD = pd.rinv;
   Q4<S> qxmC = qxm_c;
   Cell<S> uC = uc, uD = ud;
   // the next row's loads are issued one row ahead of their use
@@ -353,7 +353,11 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
     const bool fE = wave_cell_in<S>(uE.u0, uE.u1, uE.u2, uE.u3, ph);
     const Prim<S> pE = prim(uE, fE);
     Q4<S> qxmD, qxpD;
-    trace(qD, o1 ? z : slope4<S, ST1>(qC, qD, pE.q, st), cD, fC && fD && fE, qxmD, qxpD);
+    {
+      const Q4<S> dqD = o1 ? z : slope4<S, ST1>(qC, qD, pE.q, st);
+      if (fC && fD && fE) trace_cell<S, true, true>(qD, dqD, cD, tk, qxmD, qxpD, rD);
+      else trace_cell<S, false>(qD, dqD, cD, tk, qxmD, qxpD);
+    }
     const Q4<S> fcur = riemann_flux<S>(qxmC, qxpD, ph);  // face (j-2 | j-1)
     const double n0 = update1<S>(uC.u0, fprev.r, fcur.r, dtdx);
     const double n1 = update1<S>(uC.u1, fprev.u, fcur.u, dtdx);
@@ -378,6 +382,7 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
     qC = qD;
     qD = pE.q;
     cD = pE.c;
+    rD = pE.rinv;
     fC = fD;
     fD = fE;
     uC = uD;
