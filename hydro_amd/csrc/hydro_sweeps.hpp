@@ -157,7 +157,7 @@ __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
   }
   Q4<S> qxm, qxp;
   if (fc) trace_cell<S, true, true>(pc.q, dq, pc.c, tk, qxm, qxp, pc.rinv);
-  else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);
+  else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);  // (fc is false in FP32)
   const Q4<S> qr = from_right4(qxp);
   // flux through the face between this lane's cell and the next one
   const Q4<S> fr = riemann_flux<S>(qxm, qr, ph);
@@ -243,17 +243,25 @@ __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
   bool ft = fc;
   if (ph.iorder != 1) {  // launch-uniform
     lds_put<S, W>(sh.q, t, pc.q);
-    if ((t & 63) == 0) sh.in[t >> 6] = fc;
+    if constexpr (sizeof(S) == 8) {
+      if ((t & 63) == 0) sh.in[t >> 6] = fc;
+    }
     __syncthreads();
-    int all = 1;
-    for (int w = 0; w < W; w++) all &= sh.in[w];
-    ft = __builtin_amdgcn_readfirstlane(all) != 0;  // the same in every lane
+    if constexpr (sizeof(S) == 8) {
+      int all = 1;
+      for (int w = 0; w < W; w++) all &= sh.in[w];
+      ft = __builtin_amdgcn_readfirstlane(all) != 0;  // the same in every lane
+    }
     dq = slope4<S, ST1>(lds_get<S, W>(sh.q, t), pc.q, lds_get<S, W>(sh.q, t + 2),
                         ph.slope_type);
   }
   Q4<S> qxm, qxp;
-  if (ft) trace_cell<S, true, true>(pc.q, dq, pc.c, tk, qxm, qxp, pc.rinv);  // ft implies fc
-  else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);
+  if constexpr (sizeof(S) == 8) {
+    if (ft) trace_cell<S, true, true>(pc.q, dq, pc.c, tk, qxm, qxp, pc.rinv);  // ft implies fc
+    else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);
+  } else {
+    trace_cell<S>(pc.q, dq, pc.c, tk, qxm, qxp);
+  }
   lds_put<S, W>(sh.x, t, qxp);
   __syncthreads();
   const Q4<S> qr = lds_get<S, W>(sh.x, t + 2);
