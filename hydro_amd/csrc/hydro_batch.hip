@@ -65,8 +65,14 @@ __global__ __launch_bounds__(256) void k_batch_sweep_x(const InstDev<S>* __restr
     // segments tk.a + it * wpb + wave, it = 0 .. xiter-1
     SegWalk w(tk.a + wave, d.nseg);
     for (int it = 0; it < xiter && w.row < d.g.ny; it++) {
-      if (cour) x_segment<S, ST1, true>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx, cy);
-      else x_segment<S, ST1, false>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx, cy);
+      // persistent workgroups keep the SIMDs busy: the FP64 range-gated
+      // divisions pay for every instance
+      if (cour)
+        x_segment<S, ST1, true, sizeof(S) == 8>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx,
+                                                cy);
+      else
+        x_segment<S, ST1, false, sizeof(S) == 8>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx,
+                                                 cy);
       w.advance(wpb, d.nseg);
     }
     if (cour) commit_courant(cx, cy, d.cour, kInstSlots, t % kInstSlots);
@@ -98,7 +104,8 @@ __global__ __launch_bounds__(256) void k_batch_sweep_y(const InstDev<S>* __restr
     double cx = 0.0, cy = 0.0;
     if (j0 < d.g.ny + 2) {  // wave-uniform
       const int j1 = min(j0 + d.yseg, d.g.ny + 2);
-      y_march<S, ST1, false>(d.g, d.ph, tr, dtdx, in, out, colok, i, j0, j1, cx, cy);
+      y_march<S, ST1, false, sizeof(S) == 8>(d.g, d.ph, tr, dtdx, in, out, colok, i, j0, j1,
+                                             cx, cy);
     }
   }
 }

@@ -19,7 +19,7 @@ namespace hydro {
 // ---- x sweep ---------------------------------------------------------------
 
 // A workgroup walks wpb * xiter consecutive row segments, wpb at a time.
-template <typename S, bool ST1, bool COUR>
+template <typename S, bool ST1, bool COUR, bool FD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_sweep_x(GridGeom g, Phys<S> ph, double dx,
                                                  double dt_val, const RunCtl* ctl,
                                                  GridView<S> in, GridView<S> out,
@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   // segments (blockIdx.x * xiter + it) * wpb + wave, it = 0 .. xiter-1
   SegWalk w((int) blockIdx.x * xiter * wpb + wave_id(), nseg);
   for (int it = 0; it < xiter && w.row < g.ny; it++) {
-    x_segment<S, ST1, COUR>(g, ph, tk, dtdx, in, out, w.row, w.seg, cx, cy);
+    x_segment<S, ST1, COUR, FD>(g, ph, tk, dtdx, in, out, w.row, w.seg, cx, cy);
     w.advance(wpb, nseg);
   }
   if (COUR) commit_courant(cx, cy, cour, kCourSlots, blockIdx.x % kCourSlots);
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 
 // Workgroup form (W = 2 or 4 waves): a workgroup walks xiter consecutive
 // row pieces of 64*W - 4 updated cells, neighbours through LDS.
-template <typename S, bool ST1, bool COUR, int W>
+template <typename S, bool ST1, bool COUR, int W, bool FD>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) void k_sweep_xb(GridGeom g, Phys<S> ph, double dx,
                                                      double dt_val, const RunCtl* ctl,
                                                      GridView<S> in, GridView<S> out,
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   double cx = 0.0, cy = 0.0;
   SegWalk w((int) blockIdx.x * xiter, nseg);
   for (int it = 0; it < xiter && w.row < g.ny; it++) {  // workgroup-uniform
-    x_block<S, ST1, COUR, W>(g, ph, tk, dtdx, in, out, w.row, w.seg, sh, cx, cy);
+    x_block<S, ST1, COUR, W, FD>(g, ph, tk, dtdx, in, out, w.row, w.seg, sh, cx, cy);
     w.advance(1, nseg);
   }
   if (COUR) commit_courant(cx, cy, cour, kCourSlots, blockIdx.x % kCourSlots);
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
 
 // A workgroup covers 64 columns; its waves take consecutive row segments of
 // yseg rows each.
-template <typename S, bool ST1, bool COUR>
+template <typename S, bool ST1, bool COUR, bool FD>
 __global__ __launch_bounds__(256) void k_sweep_y(GridGeom g, Phys<S> ph, double dx,
                                                  double dt_val, const RunCtl* ctl,
                                                  GridView<S> in, GridView<S> out,
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) void k_sweep_y(GridGeom g, Phys<S> ph, double 
   double cx = 0.0, cy = 0.0;
   if (j0 < g.ny + 2) {  // wave-uniform
     const int j1 = min(j0 + yseg, g.ny + 2);  // rows [j0, j1) are updated
-    y_march<S, ST1, COUR>(g, ph, tk, dtdx, in, out, colok, i, j0, j1, cx, cy);
+    y_march<S, ST1, COUR, FD>(g, ph, tk, dtdx, in, out, colok, i, j0, j1, cx, cy);
   }
   if (COUR) commit_courant(cx, cy, cour, kCourSlots, blockIdx.x % kCourSlots);
 }
@@ -222,6 +222,26 @@ void launch_sweep(int idim, const GridGeom& g, const PhysArgs& pa, GridView<S> i
                                   pa.iorder, pa.scheme);
   const bool st1 = pa.slope_type == 1.0;
   const bool c = cour != nullptr;
+  // The range-gated FP64 divisions (the kernels' FD variants) pay when the
+  // launch keeps the SIMDs busy.  A sweep of a small grid has too few waves
+  // for that: each wave's march is latency-bound, and the gates' wave-wide
+  // tests, branches and larger code lengthen it, so such launches run the
+  // variants without them (the results are the same bits either way).
+  constexpr long kFastDivWaves = 4096;  // 4 waves per SIMD on 256 CUs
+  constexpr bool kFD = sizeof(S) == 8;  // FP32 storage has no FD variants
+  const bool fd = kFD && ph.fastdiv &&
+                  (idim == 1 ? (long) cdiv(g.nx, kXSeg) * g.ny
+                             : (long) cdiv(g.nx, 64) * cdiv(g.ny, 16)) >= kFastDivWaves;
+#define DISPATCH(F)                                        \
+  do {                                                     \
+    if (st1) {                                             \
+      if (c) { if (fd) F(true, true, kFD); else F(true, true, false); }    \
+      else { if (fd) F(true, false, kFD); else F(true, false, false); }   \
+    } else {                                               \
+      if (c) { if (fd) F(false, true, kFD); else F(false, true, false); }  \
+      else { if (fd) F(false, false, kFD); else F(false, false, false); } \
+    }                                                      \
+  } while (0)
   if (idim == 1 && (tune.xwaves == 2 || tune.xwaves == 4)) {
     const int W = tune.xwaves;
     const int nseg = (int) cdiv(g.nx, 64 * W - 4);
@@ -229,12 +249,11 @@ void launch_sweep(int idim, const GridGeom& g, const PhysArgs& pa, GridView<S> i
     int xiter = tune.xiter > 0 ? tune.xiter : 8;
     while (xiter > 1 && cdiv(pieces, xiter) < 1024) xiter >>= 1;
     const dim3 grid(cdiv(pieces, xiter)), blk(64 * W);
-#define HXB(A, B, WW) \
-  hipLaunchKernelGGL((k_sweep_xb<S, A, B, WW>), grid, blk, 0, st, g, ph, pa.dx, dt_val, ctl, in, \
-                     out, cour, nseg, xiter)
-#define HXW(A, B) do { if (W == 2) HXB(A, B, 2); else HXB(A, B, 4); } while (0)
-    if (st1) { if (c) HXW(true, true); else HXW(true, false); }
-    else { if (c) HXW(false, true); else HXW(false, false); }
+#define HXB(A, B, WW, F) \
+  hipLaunchKernelGGL((k_sweep_xb<S, A, B, WW, F>), grid, blk, 0, st, g, ph, pa.dx, dt_val, ctl, \
+                     in, out, cour, nseg, xiter)
+#define HXW(A, B, F) do { if (W == 2) HXB(A, B, 2, F); else HXB(A, B, 4, F); } while (0)
+    DISPATCH(HXW);
 #undef HXW
 #undef HXB
   } else if (idim == 1) {
@@ -245,11 +264,10 @@ void launch_sweep(int idim, const GridGeom& g, const PhysArgs& pa, GridView<S> i
     // keep at least ~4 workgroups per CU before walking several segments
     while (xiter > 1 && cdiv(waves, (long) xw * xiter) < 1024) xiter >>= 1;
     const dim3 grid(cdiv(waves, (long) xw * xiter)), blk(64 * xw);
-#define HX(A, B) \
-  hipLaunchKernelGGL((k_sweep_x<S, A, B>), grid, blk, 0, st, g, ph, pa.dx, dt_val, ctl, in, out, \
-                     cour, nseg, xiter)
-    if (st1) { if (c) HX(true, true); else HX(true, false); }
-    else { if (c) HX(false, true); else HX(false, false); }
+#define HX(A, B, F) \
+  hipLaunchKernelGGL((k_sweep_x<S, A, B, F>), grid, blk, 0, st, g, ph, pa.dx, dt_val, ctl, in, \
+                     out, cour, nseg, xiter)
+    DISPATCH(HX);
 #undef HX
   } else {
     int yseg = tune.yseg > 0 ? tune.yseg : 64;
@@ -259,16 +277,16 @@ void launch_sweep(int idim, const GridGeom& g, const PhysArgs& pa, GridView<S> i
     // (down to 16 rows; each adds a 4-row halo) until there are about two
     // waves per SIMD, rather than leave most of the GPU idle
     while (yseg > 16 && (long) ncolblk * cdiv(g.ny, yseg) < 2048) yseg >>= 1;
-#define HY(A, B)                                                                               \
+#define HY(A, B, F)                                                                            \
   do {                                                                                         \
     const int nsg = (int) cdiv(g.ny, (long) yseg * yw);                                        \
-    hipLaunchKernelGGL((k_sweep_y<S, A, B>), dim3((unsigned) ncolblk * nsg), dim3(64 * yw), 0, \
-                       st, g, ph, pa.dx, dt_val, ctl, in, out, cour, yseg, ncolblk);           \
+    hipLaunchKernelGGL((k_sweep_y<S, A, B, F>), dim3((unsigned) ncolblk * nsg), dim3(64 * yw),  \
+                       0, st, g, ph, pa.dx, dt_val, ctl, in, out, cour, yseg, ncolblk);        \
   } while (0)
-    if (st1) { if (c) HY(true, true); else HY(true, false); }
-    else { if (c) HY(false, true); else HY(false, false); }
+    DISPATCH(HY);
 #undef HY
   }
+#undef DISPATCH
 }
 
 template <typename S>

@@ -323,13 +323,14 @@ __device__ __forceinline__ bool wave_cell_in(S U0, S U1, S U2, S U3, const Phys<
 }
 
 // Courant numbers of one cell in grid slot order (compute_deltat.c:71-88 +
-// courantOnXY.c:115-121): returns c + |vx| and c + |vy|.  The primitives take
-// FastDiv when every active lane's cell passes cell_in.
-template <typename R>
+// courantOnXY.c:115-121): returns c + |vx| and c + |vy|.  FD: the primitives
+// take FastDiv when every active lane's cell passes cell_in.
+template <typename R, bool FD = false>
 __device__ __forceinline__ void courant_cell(R U0, R U1, R U2, R U3, const Phys<R>& ph, R& cx,
                                              R& cy) {
-  const Prim<R> o = wave_cell_in<R>(U0, U1, U2, U3, ph) ? prim_cell<R, true>(U0, U1, U2, U3, ph)
-                                                        : prim_cell<R, false>(U0, U1, U2, U3, ph);
+  const Prim<R> o = FD && wave_cell_in<R>(U0, U1, U2, U3, ph)
+                        ? prim_cell<R, true>(U0, U1, U2, U3, ph)
+                        : prim_cell<R, false>(U0, U1, U2, U3, ph);
   cx = o.c + fabs(o.q.u);
   cy = o.c + fabs(o.q.v);
 }
@@ -553,14 +554,16 @@ __device__ __forceinline__ Q4<double> star_flux(double pst, const Q4<double>& ql
 // block takes FastDiv when its operands are bounded -- the floored inputs
 // passed their tests (ff) and, for a Newton step and the star state, the
 // wave's pressures pass pst_in -- and IeeeDiv otherwise.
+template <bool G>
 __device__ __forceinline__ Q4<double> riemann_d(const Q4<double>& ql, const Q4<double>& qr,
                                                 double rl, double ul, double pl, double rr,
                                                 double ur, double pr, double cl, double cr,
                                                 bool ff, const Phys<double>& ph) {
+  ff = G && ff;  // G false: the IEEE code alone
   const double g6 = ph.gamma6_d;
   double wl0, wr0, pst;
   Rcp rpl{pl, 0.0}, rpr{pr, 0.0};
-  if (ff) {
+  if (G && ff) {
     FastDiv::sqrt2(cl, cr, wl0, wr0);
     pst = fmax(FastDiv::div((wr0 * pl + wl0 * pr) + wl0 * wr0 * (ul - ur), wl0 + wr0), 0.0);
     rpl = rcp_of(pl);
@@ -571,14 +574,14 @@ __device__ __forceinline__ Q4<double> riemann_d(const Q4<double>& ql, const Q4<d
   }
   for (int it = 0; it < ph.niter; it++) {
     bool done;
-    if (ff && wave_all(pst_in(pst)))
+    if (G && ff && wave_all(pst_in(pst)))
       done = newton_step<FastDiv>(pst, pl, pr, cl, cr, ul, ur, rpl, rpr, g6, ph.smallpp_d);
     else
       done = newton_step<IeeeDiv>(pst, pl, pr, cl, cr, ul, ur, IeeeDiv::R{pl}, IeeeDiv::R{pr},
                                   g6, ph.smallpp_d);
     if (done) break;
   }
-  if (ff && wave_all(pst_in(pst)))
+  if (G && ff && wave_all(pst_in(pst)))
     return star_flux<FastDiv>(pst, ql, qr, rl, ul, pl, rr, ur, pr, cl, cr, rpl, rpr, ph);
   return star_flux<IeeeDiv>(pst, ql, qr, rl, ul, pl, rr, ur, pr, cl, cr, IeeeDiv::R{pl},
                             IeeeDiv::R{pr}, ph);
@@ -682,8 +685,9 @@ __device__ __forceinline__ Q4<R> riemann_flux_mixed(const Q4<R>& ql, const Q4<R>
 }
 
 // riemann.c:128-276 for one face (qleft = ql, qright = qr) followed by
-// cmpflx.c:67-90: returns the Godunov flux through the face.
-template <typename R>
+// cmpflx.c:67-90: returns the Godunov flux through the face.  FD (FP64): the
+// range-gated divisions may be used; without it the IEEE code alone.
+template <typename R, bool FD = false>
 __device__ __forceinline__ Q4<R> riemann_flux(const Q4<R>& ql, const Q4<R>& qr,
                                               const Phys<R>& ph) {
   if constexpr (sizeof(R) == 8) {
@@ -695,10 +699,14 @@ __device__ __forceinline__ Q4<R> riemann_flux(const Q4<R>& ql, const Q4<R>& qr,
     const double pr = fmax(qr.p, rr * ph.smallp);
     const double cl = ph.gamma * pl * rl;
     const double cr = ph.gamma * pr * rr;
-    // the face's floored densities and pressures in_pos, velocities in_zero
-    const bool fast = ph.fastdiv && wave_all(in_pos(rl) && in_pos(pl) && in_pos(rr) &&
-                                             in_pos(pr) && in_zero(ul) && in_zero(ur));
-    return riemann_d(ql, qr, rl, ul, pl, rr, ur, pr, cl, cr, fast, ph);
+    if constexpr (FD) {
+      // the face's floored densities and pressures in_pos, velocities in_zero
+      const bool fast = ph.fastdiv && wave_all(in_pos(rl) && in_pos(pl) && in_pos(rr) &&
+                                               in_pos(pr) && in_zero(ul) && in_zero(ur));
+      return riemann_d<true>(ql, qr, rl, ul, pl, rr, ur, pr, cl, cr, fast, ph);
+    } else {
+      return riemann_d<false>(ql, qr, rl, ul, pl, rr, ur, pr, cl, cr, false, ph);
+    }
   } else {
     return riemann_flux_mixed(ql, qr, ph);
   }

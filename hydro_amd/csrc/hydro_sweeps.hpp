@@ -134,7 +134,9 @@ struct SegWalk {
   }
 };
 
-template <typename S, bool ST1, bool COUR>
+// FD: the range-gated FP64 divisions may be used (FP64 launches with enough
+// waves to keep the SIMDs busy); without it the IEEE code alone.
+template <typename S, bool ST1, bool COUR, bool FD>
 __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
                                           const TraceK<S>& tk, double dtdx,
                                           const GridView<S>& in, const GridView<S>& out,
@@ -147,7 +149,7 @@ __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
 
   const S U0 = in.p[0][o], U1 = in.p[1][o], U2 = in.p[2][o], U3 = in.p[3][o];
   // every cell of the wave (the slope neighbours included) in range: FastDiv
-  const bool fc = wave_cell_in<S>(U0, U1, U2, U3, ph);
+  const bool fc = FD && wave_cell_in<S>(U0, U1, U2, U3, ph);
   const Prim<S> pc = fc ? prim_cell<S, true>(U0, U1, U2, U3, ph)
                         : prim_cell<S, false>(U0, U1, U2, U3, ph);
   Q4<S> dq = {(S) 0, (S) 0, (S) 0, (S) 0};
@@ -160,7 +162,7 @@ __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
   else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);  // (fc is false in FP32)
   const Q4<S> qr = from_right4(qxp);
   // flux through the face between this lane's cell and the next one
-  const Q4<S> fr = riemann_flux<S>(qxm, qr, ph);
+  const Q4<S> fr = riemann_flux<S, FD>(qxm, qr, ph);
   const Q4<S> fl = from_left4(fr);
 
   const bool wr = lane >= 2 && lane <= kXSeg + 1 && i <= g.nx + 1;
@@ -175,7 +177,7 @@ __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
     out.p[3][o] = (S) n3;
     if (COUR) {
       S ax, ay;
-      courant_cell<S>((S) n0, (S) n1, (S) n2, (S) n3, ph, ax, ay);
+      courant_cell<S, FD>((S) n0, (S) n1, (S) n2, (S) n3, ph, ax, ay);
       cx = maxm(cx, (double) ax);
       cy = maxm(cy, (double) ay);
     }
@@ -220,7 +222,7 @@ __device__ __forceinline__ Q4<S> lds_get(const S (&b)[4][64 * W + 2], int slot) 
 // once per wave.  Every thread of the workgroup calls it with the same
 // (jrow, seg): it has barriers.  Consecutive calls may reuse `sh`: each
 // buffer is rewritten only after a barrier that follows its last read.
-template <typename S, bool ST1, bool COUR, int W>
+template <typename S, bool ST1, bool COUR, int W, bool FD>
 __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
                                         const TraceK<S>& tk, double dtdx,
                                         const GridView<S>& in, const GridView<S>& out,
@@ -236,18 +238,18 @@ __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
   const S U0 = in.p[0][o], U1 = in.p[1][o], U2 = in.p[2][o], U3 = in.p[3][o];
   // FastDiv for the primitives when the wave's cells are in range, for the
   // traces when every wave's are (the slopes read neighbours of other waves)
-  const bool fc = wave_cell_in<S>(U0, U1, U2, U3, ph);
+  const bool fc = FD && wave_cell_in<S>(U0, U1, U2, U3, ph);
   const Prim<S> pc = fc ? prim_cell<S, true>(U0, U1, U2, U3, ph)
                         : prim_cell<S, false>(U0, U1, U2, U3, ph);
   Q4<S> dq = {(S) 0, (S) 0, (S) 0, (S) 0};
   bool ft = fc;
   if (ph.iorder != 1) {  // launch-uniform
     lds_put<S, W>(sh.q, t, pc.q);
-    if constexpr (sizeof(S) == 8) {
+    if constexpr (FD) {
       if ((t & 63) == 0) sh.in[t >> 6] = fc;
     }
     __syncthreads();
-    if constexpr (sizeof(S) == 8) {
+    if constexpr (FD) {
       int all = 1;
       for (int w = 0; w < W; w++) all &= sh.in[w];
       ft = __builtin_amdgcn_readfirstlane(all) != 0;  // the same in every lane
@@ -256,7 +258,7 @@ __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
                         ph.slope_type);
   }
   Q4<S> qxm, qxp;
-  if constexpr (sizeof(S) == 8) {
+  if constexpr (FD) {
     if (ft) trace_cell<S, true, true>(pc.q, dq, pc.c, tk, qxm, qxp, pc.rinv);  // ft implies fc
     else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);
   } else {
@@ -266,7 +268,7 @@ __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
   __syncthreads();
   const Q4<S> qr = lds_get<S, W>(sh.x, t + 2);
   // flux through the face between this thread's cell and the next one
-  const Q4<S> fr = riemann_flux<S>(qxm, qr, ph);
+  const Q4<S> fr = riemann_flux<S, FD>(qxm, qr, ph);
   lds_put<S, W>(sh.f, t, fr);
   __syncthreads();
   const Q4<S> fl = lds_get<S, W>(sh.f, t);
@@ -283,7 +285,7 @@ __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
     out.p[3][o] = (S) n3;
     if (COUR) {
       S ax, ay;
-      courant_cell<S>((S) n0, (S) n1, (S) n2, (S) n3, ph, ax, ay);
+      courant_cell<S, FD>((S) n0, (S) n1, (S) n2, (S) n3, ph, ax, ay);
       cx = maxm(cx, (double) ax);
       cy = maxm(cy, (double) ay);
     }
@@ -303,7 +305,7 @@ __device__ __forceinline__ Cell<S> load_y(const GridView<S>& in, long o) {
   return Cell<S>{in.p[0][o], in.p[2][o], in.p[1][o], in.p[3][o]};
 }
 
-template <typename S, bool ST1, bool COUR>
+template <typename S, bool ST1, bool COUR, bool FD>
 __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
                                         const TraceK<S>& tk, double dtdx,
                                         const GridView<S>& in, const GridView<S>& out,
@@ -328,10 +330,10 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
     if (f) trace_cell<S, true>(q, dq, c, tk, qxm, qxp);
     else trace_cell<S, false>(q, dq, c, tk, qxm, qxp);
   };
-  const bool fa = wave_cell_in<S>(ua.u0, ua.u1, ua.u2, ua.u3, ph);
-  const bool fb = wave_cell_in<S>(ub.u0, ub.u1, ub.u2, ub.u3, ph);
-  bool fC = wave_cell_in<S>(uc.u0, uc.u1, uc.u2, uc.u3, ph);
-  bool fD = wave_cell_in<S>(ud.u0, ud.u1, ud.u2, ud.u3, ph);
+  const bool fa = FD && wave_cell_in<S>(ua.u0, ua.u1, ua.u2, ua.u3, ph);
+  const bool fb = FD && wave_cell_in<S>(ub.u0, ub.u1, ub.u2, ub.u3, ph);
+  bool fC = FD && wave_cell_in<S>(uc.u0, uc.u1, uc.u2, uc.u3, ph);
+  bool fD = FD && wave_cell_in<S>(ud.u0, ud.u1, ud.u2, ud.u3, ph);
   Prim<S> pa = prim(ua, fa);
   Prim<S> pb = prim(ub, fb);
   Prim<S> pcc = prim(uc, fC);
@@ -342,7 +344,7 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
         qxp_b);
   trace(pcc.q, o1 ? z : slope4<S, ST1>(pb.q, pcc.q, pd.q, st), pcc.c, fb && fC && fD, qxm_c,
         qxp_c);
-  Q4<S> fprev = riemann_flux<S>(qxm_b, qxp_c, ph);  // face (j0-1 | j0)
+  Q4<S> fprev = riemann_flux<S, FD>(qxm_b, qxp_c, ph);  // face (j0-1 | j0)
 
   // window: cell C = row j-2 (being updated), D = row j-1; carried: qxm of C,
   // flux through the face below C
@@ -358,7 +360,7 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
     const Cell<S> uE = uN;
     const int jn = j + 1 <= j1 + 1 ? j + 1 : j1 + 1;  // last row: a harmless reload
     uN = load_y(in, (long) jn * P + i);
-    const bool fE = wave_cell_in<S>(uE.u0, uE.u1, uE.u2, uE.u3, ph);
+    const bool fE = FD && wave_cell_in<S>(uE.u0, uE.u1, uE.u2, uE.u3, ph);
     const Prim<S> pE = prim(uE, fE);
     Q4<S> qxmD, qxpD;
     {
@@ -366,7 +368,7 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
       if (fC && fD && fE) trace_cell<S, true, true>(qD, dqD, cD, tk, qxmD, qxpD, rD);
       else trace_cell<S, false>(qD, dqD, cD, tk, qxmD, qxpD);
     }
-    const Q4<S> fcur = riemann_flux<S>(qxmC, qxpD, ph);  // face (j-2 | j-1)
+    const Q4<S> fcur = riemann_flux<S, FD>(qxmC, qxpD, ph);  // face (j-2 | j-1)
     const double n0 = update1<S>(uC.u0, fprev.r, fcur.r, dtdx);
     const double n1 = update1<S>(uC.u1, fprev.u, fcur.u, dtdx);
     const double n2 = update1<S>(uC.u2, fprev.v, fcur.v, dtdx);
@@ -380,7 +382,7 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
       out.p[3][oc] = (S) n3;
       if (COUR) {
         S ax, ay;
-        courant_cell<S>((S) n0, (S) n2, (S) n1, (S) n3, ph, ax, ay);
+        courant_cell<S, FD>((S) n0, (S) n2, (S) n1, (S) n3, ph, ax, ay);
         cx = maxm(cx, (double) ax);
         cy = maxm(cy, (double) ay);
       }
