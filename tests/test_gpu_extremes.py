@@ -32,19 +32,40 @@ def params(nx, ny, **kw):
     return hydro_amd.Params(**base)
 
 
-def run_both(p, u0, steps):
-    with hydro_amd.Grid(p) as g:
-        g.upload(u0)
-        dts = g.run(steps)
-        u = g.download()
+def run_both(p, u0, steps, form="grid"):
+    """The state after `steps` steps on the GPU and in the oracle.  form
+    "grid": a single grid (these small grids take the sweep variants without
+    the range-gated divisions); "batch": a one-instance batch, whose
+    persistent kernels always carry them, so the gates' fast and IEEE
+    branches both run here."""
+    if form == "grid":
+        with hydro_amd.Grid(p) as g:
+            g.upload(u0)
+            dts = g.run(steps)
+            u = g.download()
+    else:
+        with hydro_amd.Batch(hydro_amd.FP64) as b:
+            i = b.add(p)
+            b.upload(i, u0)
+            b.run(steps)
+            u = b.download(i)
+            t, n, dt, _ = b.query()[i]
+        dts = None
     port = oracle.Port()
     ur = u0.copy()
-    rdts, _ = port.run(to_oracle(p), ur, steps)
+    rdts, state = port.run(to_oracle(p), ur, steps)
+    if dts is None:  # the batch reports the run's end state, not each dt
+        assert (t, n) == (state[0], state[1]) or (np.isnan(t) and np.isnan(state[0]))
+        dts = rdts
     return u, dts, ur, rdts
 
 
+FORMS = ["grid", "batch"]
+
+
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("slope_type", [1.0, 1.5])
-def test_tiny_scale_state(slope_type):
+def test_tiny_scale_state(slope_type, form):
     """A smooth blast scaled to rho ~ p ~ 1e-120 (smallr = smallc = 1e-200):
     every Riemann square root (gamma * p * rho ~ 1e-240) is below 2^-767, so
     the general sequence runs everywhere; the state stays finite."""
@@ -55,13 +76,15 @@ def test_tiny_scale_state(slope_type):
     u[0, 2:-2, 2:-2] = s * (1 + 0.1 * rng.random((70, 90)))
     u[3, 2:-2, 2:-2] = s * (2.5 + 0.1 * rng.random((70, 90)))
     u[3, 30:34, 40:44] = s * 25.0
-    g, dts, r, rdts = run_both(p, u, 8)
+    g, dts, r, rdts = run_both(p, u, 8, form)
     assert len(dts) == 8 and np.isfinite(dts).all() and np.isfinite(g).all()
     assert np.array_equal(dts, rdts), (dts, rdts)
     assert np.array_equal(g, r), f"{np.count_nonzero(g != r)} values differ"
 
 
-def test_near_vacuum_blow_up():
+def test_near_vacuum_blow_up(form="grid"):
+    # (a batch stops an instance whose dt stops being finite, which the
+    # oracle's run does not, so this breakdown is compared on a single grid)
     """Densities spread over 250 decades next to a blast: the run breaks down
     into NaNs within a few steps, identically (NaN matching NaN)."""
     p = params(90, 70, smallr=1e-300, smallc=1e-300)
@@ -73,13 +96,13 @@ def test_near_vacuum_blow_up():
     u[3, 2:-2, 2:-2] = 10.0 ** rng.uniform(-290, -200, (70, 90)) + 0.5 * (
         u[1, 2:-2, 2:-2] ** 2 + u[2, 2:-2, 2:-2] ** 2) / u[0, 2:-2, 2:-2]
     u[3, 30:34, 40:44] = 1e5
-    g, dts, r, rdts = run_both(p, u, 8)
+    g, dts, r, rdts = run_both(p, u, 8, form)
     assert len(dts) >= 2
     assert same(dts, rdts), (dts, rdts)
     assert same(g, r), f"{np.count_nonzero(~((g == r) | (np.isnan(g) & np.isnan(r))))} differ"
 
 
-def test_huge_and_zero_values():
+def test_huge_and_zero_values(form="grid"):
     """Energies near 1e300, exact zeros in density and energy, smallc = 0."""
     p = params(64, 48, smallc=0.0, smallr=1e-10)
     u = np.zeros(p.shape())
@@ -87,14 +110,15 @@ def test_huge_and_zero_values():
     u[0, 2:-2, 2:-2] = rng.choice([0.0, 1e-300, 1.0, 1e150], size=(48, 64))
     u[3, 2:-2, 2:-2] = rng.choice([0.0, 1e-310, 1e-5, 1e300], size=(48, 64))
     u[1, 2:-2, 2:-2] = rng.choice([0.0, -0.0, 1e-200, -3.0], size=(48, 64))
-    g, dts, r, rdts = run_both(p, u, 3)
+    g, dts, r, rdts = run_both(p, u, 3, form)
     assert len(dts) >= 1
     assert same(dts, rdts), (dts, rdts)
     assert same(g, r)
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("seed", [21, 22])
-def test_division_gate_boundaries(seed):
+def test_division_gate_boundaries(seed, form):
     """Densities, pressures and momenta scattered just inside and just outside
     the division tests' bounds (2^-41, 2^40), in some columns and rows only:
     waves and faces of one sweep take the fast and the IEEE divisions side by
@@ -114,6 +138,6 @@ def test_division_gate_boundaries(seed):
     u[1, 32:47, 2:-2] = lo * rng.uniform(-1, 1, (15, 150))
     u[3, 32:47, 2:-2] = lo * rng.uniform(0.5, 2.0, (15, 150))
     u[3, 70:74, 100:104] = 1e6
-    g, dts, r, rdts = run_both(p, u, 6)
+    g, dts, r, rdts = run_both(p, u, 6, form)
     assert np.array_equal(dts, rdts), (dts, rdts)
     assert same(g, r), f"{np.count_nonzero(~((g == r) | (np.isnan(g) & np.isnan(r))))} differ"
