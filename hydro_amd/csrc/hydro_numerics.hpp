@@ -255,6 +255,22 @@ __device__ __forceinline__ double sqrt_nr(double x) {
   const double v = __builtin_fma(s, r, s);
   return x == 0.0 ? x : v;  // rsq(0) = inf
 }
+// 1/b: the v_rcp_f64 estimate refined by two Newton-Raphson steps.
+__device__ __forceinline__ double rcp_nr(double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  y = __builtin_fma(__builtin_fma(-b, y, 1.0), y, y);
+  return __builtin_fma(__builtin_fma(-b, y, 1.0), y, y);
+}
+// sqrt(x) and 1/sqrt(x) for x > 0 from one v_rsq_f64 estimate g: with
+// s = x g and the residual r = 1/2 - s g/2, one Goldschmidt step gives
+// s + s r and g + g r.
+__device__ __forceinline__ void sqrt_rsq_nr(double x, double& s, double& is) {
+  const double g = __builtin_amdgcn_rsq(x);
+  const double s0 = x * g;
+  const double r = __builtin_fma(-(g * 0.5), s0, 0.5);
+  s = __builtin_fma(s0, r, s0);
+  is = __builtin_fma(g, r, g);
+}
 template <typename R>
 __device__ __forceinline__ double ndiv(double a, double b);
 template <>
@@ -605,26 +621,31 @@ __device__ __forceinline__ Q4<R> riemann_flux_mixed(const Q4<R>& ql, const Q4<R>
   R cr = ph.gamma * pr * rr;
 
   // First guess and Newton iteration for p* (riemann.c:145-201), in double.
+  // The iteration is the reference's recurrence rearranged to fewer
+  // divisions: 1/pl and 1/pr are loop invariants; 1/w comes from the same
+  // reciprocal square root as w; with ql = al/bl and qr = ar/br (al = 2 wl^3,
+  // bl = wl^2 + cl), qr ql / (qr + ql) = al ar / (al br + ar bl) is one
+  // division; |d / (p* + smallpp)| > 1e-6 is tested as |d| > 1e-6 (p* +
+  // smallpp) (p* >= 0, smallpp > 0).
   const double dpl = pl, dpr = pr, dcl = cl, dcr = cr, dul = ul, dur = ur;
   double wl0, wr0;
   nsqrt2<R>(dcl, dcr, wl0, wr0);
   double pst = fmax(ndiv<R>((wr0 * dpl + wl0 * dpr) + wl0 * wr0 * (dul - dur), wl0 + wr0), 0.0);
   const double g6 = ph.gamma6_d;
+  const double gpl = g6 * rcp_nr(dpl), gpr = g6 * rcp_nr(dpr);
   for (int it = 0; it < ph.niter; it++) {
-    double wwl, wwr;
-    nsqrt2<R>(dcl * (1.0 + ndiv<R>(g6 * (pst - dpl), dpl)),
-              dcr * (1.0 + ndiv<R>(g6 * (pst - dpr), dpr)), wwl, wwr);
-    double swwl = wwl * wwl;
-    double qlw = ndiv<R>(2.0 * wwl * swwl, swwl + dcl);
-    double qrw = ndiv<R>(2.0 * wwr * (wwr * wwr), (wwr * wwr) + dcr);
-    double usl = dul - ndiv<R>(pst - dpl, wwl);
-    double usr = dur + ndiv<R>(pst - dpr, wwr);
-    double t = (ndiv<R>(qrw * qlw, qrw + qlw) * (usl - usr));
-    double d = (t > (-pst)) ? t : (-pst);
+    double wwl, iwl, wwr, iwr;
+    sqrt_rsq_nr(dcl * (1.0 + (pst - dpl) * gpl), wwl, iwl);
+    sqrt_rsq_nr(dcr * (1.0 + (pst - dpr) * gpr), wwr, iwr);
+    const double swl = wwl * wwl, swr = wwr * wwr;
+    const double al = 2.0 * wwl * swl, bl = swl + dcl;
+    const double ar = 2.0 * wwr * swr, br = swr + dcr;
+    const double usl = dul - (pst - dpl) * iwl;
+    const double usr = dur + (pst - dpr) * iwr;
+    const double t = div_nr(al * ar, al * br + ar * bl) * (usl - usr);
+    const double d = (t > (-pst)) ? t : (-pst);
     pst += d;
-    double rel = ndiv<R>(d, pst + ph.smallpp_d);
-    double arel = (rel > 0) ? rel : -rel;
-    if (!(arel > 1e-6)) break;
+    if (!(fabs(d) > 1e-6 * (pst + ph.smallpp_d))) break;
   }
 
   // Star state and sampling at x/t = 0 (riemann.c:211-276).
