@@ -427,10 +427,13 @@ __device__ __forceinline__ void trace_cell(const Q4<R>& q, const Q4<R>& dq, R cc
     dprcc = P::div(dq.p, q.r * cc);
     a0r = dq.r - P::div(dq.p, csq);
   } else {
-    rOcc = xdiv(q.r, cc);
-    OrOcc = xdiv(cc, q.r);
-    dprcc = xdiv(dq.p, q.r * cc);
-    a0r = sub_div(dq.r, dq.p, csq);
+    // two reciprocals for the four quotients
+    const R rcc = __builtin_amdgcn_rcpf(cc);
+    const R rr = __builtin_amdgcn_rcpf(q.r);
+    rOcc = q.r * rcc;
+    OrOcc = cc * rr;
+    dprcc = (dq.p * rr) * rcc;
+    a0r = dq.r - (dq.p * rcc) * rcc;
   }
   R am = half * (dprcc - dq.u) * rOcc;
   R ap = half * (dprcc + dq.u) * rOcc;
@@ -649,19 +652,22 @@ __device__ __forceinline__ Q4<R> riemann_flux_mixed(const Q4<R>& ql, const Q4<R>
   }
 
   // Star state and sampling at x/t = 0 (riemann.c:211-276).
+  // (g6/pl, g6/pr and 1/wl, 1/wr shared with the Newton iteration and rstar)
   R ps = (R) pst;
   R wr, wl;
-  xsqrt2(cr * add_div(one, (R) g6 * (ps - pr), pr), cl * add_div(one, (R) g6 * (ps - pl), pl),
+  xsqrt2(cr * __builtin_fmaf(ps - pr, (R) gpr, one), cl * __builtin_fmaf(ps - pl, (R) gpl, one),
          wr, wl);
+  const R iwl = __builtin_amdgcn_rcpf(wl), iwr = __builtin_amdgcn_rcpf(wr);
   // ((ul + (pl-ps)/wl) + ur) - (pr-ps)/wr, the reference's left-to-right sum
-  R ustar = half * sub_div(add_div(ul, pl - ps, wl) + ur, pr - ps, wr);
+  R ustar = half * __builtin_fmaf(-(pr - ps), iwr, __builtin_fmaf(pl - ps, iwl, ul) + ur);
   bool left = ustar > (R) 0;
   R sg = left ? (R) 1 : (R) -1;
   R ro = left ? rl : rr;
   R uo = left ? ul : ur;
   R po = left ? pl : pr;
   R wo = left ? wl : wr;
-  R rstar = xdiv(ro, add_div(one, ro * (po - ps), wo * wo));
+  R iwo = left ? iwl : iwr;
+  R rstar = xdiv(ro, __builtin_fmaf(ro * (po - ps), iwo * iwo, one));
   rstar = fmax(rstar, ph.smallr);
   R co, cstar;  // (riemann.c computes co before rstar; neither depends on the other)
   xsqrt2(fabs(xdiv(ph.gamma * po, ro)), fabs(xdiv(ph.gamma * ps, rstar)), co, cstar);
