@@ -352,7 +352,8 @@ def measured_valu(nx, precision, tune, kern, avg_s):
     """VALU work of the sweep kernel `kern` from one rocprofv3 --pmc pass over
     pmc_workload: VALU instructions per cell, FP64 FLOP/s (FMA = 2) over the
     kernel's measured duration `avg_s`, and the share of SIMD cycles issuing
-    VALU (each wave64 instruction holds a 16-lane SIMD for 4 cycles)."""
+    VALU, counting each wave64 instruction at 4 cycles (a 16-lane SIMD; the
+    FP32-heavy mixed form reads above 1, so some instructions take fewer)."""
     args = ["--steps", "2", "--nx", str(nx), "--precision", ["fp64", "mixed"][precision]]
     if tune:
         args += ["--tune", ",".join(str(tune[k]) for k in ("yseg", "ywaves", "xwaves", "xiter"))]
@@ -370,7 +371,10 @@ def measured_valu(nx, precision, tune, kern, avg_s):
             "fp64_tflops": round(flops / avg_s / 1e12, 2),
             "fp64_peak_tflops": FP64_VALU_PEAK / 1e12,
             "fp64_frac": round(flops / avg_s / FP64_VALU_PEAK, 4),
-            "valu_issue_busy": round(v["SQ_INSTS_VALU"] * 4 / (SIMDS * cycles), 3)}
+            "valu_issue_busy": round(v["SQ_INSTS_VALU"] * 4 / (SIMDS * cycles), 3),
+            "valu_issue_note": "SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x GPU cycles): every "
+                               "wave64 VALU instruction counted at 4 cycles; above 1 where some "
+                               "instructions issue in fewer"}
 
 
 def main():
