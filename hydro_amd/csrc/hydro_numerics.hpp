@@ -151,6 +151,16 @@ __device__ __forceinline__ double qdiv(double a, const Rcp& d) {
   const double q = a * d.r;
   return __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-d.b, q, a), d.r, q), d.b, a);
 }
+// qdiv without the v_div_fixup, for a numerator that is never -0.  For
+// finite, nonzero, in-range operands the fix-up returns the core's quotient
+// with the sign sign(a) * sign(b), which the core's nonzero result already
+// has; for a = +0 (b > 0) the core gives +0 itself (+0 * r, then
+// fma(-b, +0, +0) = +0, fma(+0, r, +0) = +0).  Only a = -0, where the
+// fix-up returns -0 and the core +0, needs it.
+__device__ __forceinline__ double qdiv_nz(double a, const Rcp& d) {
+  const double q = a * d.r;
+  return __builtin_fma(__builtin_fma(-d.b, q, a), d.r, q);
+}
 
 // Division / square-root policies of the FP64 numerics.  sqrt1/sqrt2 of
 // FastDiv take the sqrt_core sequence directly: callers use them only for
@@ -162,6 +172,9 @@ struct IeeeDiv {
   static __device__ __forceinline__ R rcp(double b) { return R{b}; }
   static __device__ __forceinline__ double q(double a, const R& d) { return a / d.b; }
   static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+  // (numerators that are never -0: see qdiv_nz)
+  static __device__ __forceinline__ double qn(double a, const R& d) { return a / d.b; }
+  static __device__ __forceinline__ double divn(double a, double b) { return a / b; }
   static __device__ __forceinline__ double sqrt1(double x) { return sqrt_fast(x); }
   static __device__ __forceinline__ void sqrt2(double x, double y, double& sx, double& sy) {
     sqrt_fast2(x, y, sx, sy);
@@ -172,6 +185,10 @@ struct FastDiv {
   static __device__ __forceinline__ R rcp(double b) { return rcp_of(b); }
   static __device__ __forceinline__ double q(double a, const R& d) { return qdiv(a, d); }
   static __device__ __forceinline__ double div(double a, double b) { return qdiv(a, rcp_of(b)); }
+  static __device__ __forceinline__ double qn(double a, const R& d) { return qdiv_nz(a, d); }
+  static __device__ __forceinline__ double divn(double a, double b) {
+    return qdiv_nz(a, rcp_of(b));
+  }
   static __device__ __forceinline__ double sqrt1(double x) { return sqrt_core(x); }
   static __device__ __forceinline__ void sqrt2(double x, double y, double& sx, double& sy) {
     sx = sqrt_core(x);
@@ -310,7 +327,7 @@ __device__ __forceinline__ Prim<R> prim_cell(R U0, R U1, R U2, R U3, const Phys<
     e = P::q(U3, rr) - eken;
     p = ph.gm1 * r * e;
     p = maxm(p, r * ph.smallp);
-    o.c = P::sqrt1(P::q(ph.gamma * p, rr));
+    o.c = P::sqrt1(P::qn(ph.gamma * p, rr));  // gamma*p > 0
   } else {
     u = xdiv(U1, r);
     v = xdiv(U2, r);
@@ -421,9 +438,9 @@ __device__ __forceinline__ void trace_cell(const Q4<R>& q, const Q4<R>& dq, R cc
   R rOcc, OrOcc, dprcc, a0r;
   if constexpr (sizeof(R) == 8) {
     using P = std::conditional_t<FAST, FastDiv, IeeeDiv>;
-    rOcc = P::div(q.r, cc);
-    if constexpr (FAST && RINV) OrOcc = qdiv(cc, Rcp{q.r, rinv});
-    else OrOcc = P::div(cc, q.r);
+    rOcc = P::divn(q.r, cc);  // rho, c > 0
+    if constexpr (FAST && RINV) OrOcc = qdiv_nz(cc, Rcp{q.r, rinv});
+    else OrOcc = P::divn(cc, q.r);
     dprcc = P::div(dq.p, q.r * cc);
     a0r = dq.r - P::div(dq.p, csq);
   } else {
@@ -480,18 +497,21 @@ __device__ __forceinline__ bool newton_step(double& pst, double pl, double pr, d
                                             double cr, double ul, double ur,
                                             const typename P::R& rpl, const typename P::R& rpr,
                                             double g6, double smallpp) {
+  // Numerators: pst - p is +0 when pst == p (never -0, p > 0); the wave
+  // speeds and q's are positive; rel (which may be +-0 for d = +-0) is only
+  // compared by magnitude.
   double wwl, wwr;
-  P::sqrt2(cl * (1.0 + P::q(g6 * (pst - pl), rpl)), cr * (1.0 + P::q(g6 * (pst - pr), rpr)),
+  P::sqrt2(cl * (1.0 + P::qn(g6 * (pst - pl), rpl)), cr * (1.0 + P::qn(g6 * (pst - pr), rpr)),
            wwl, wwr);
   const double swwl = wwl * wwl;
-  const double qlw = P::div(2.0 * wwl * swwl, swwl + cl);
-  const double qrw = P::div(2.0 * wwr * (wwr * wwr), (wwr * wwr) + cr);
-  const double usl = ul - P::div(pst - pl, wwl);
-  const double usr = ur + P::div(pst - pr, wwr);
-  const double t = (P::div(qrw * qlw, qrw + qlw) * (usl - usr));
+  const double qlw = P::divn(2.0 * wwl * swwl, swwl + cl);
+  const double qrw = P::divn(2.0 * wwr * (wwr * wwr), (wwr * wwr) + cr);
+  const double usl = ul - P::divn(pst - pl, wwl);
+  const double usr = ur + P::divn(pst - pr, wwr);
+  const double t = (P::divn(qrw * qlw, qrw + qlw) * (usl - usr));
   const double d = (t > (-pst)) ? t : (-pst);
   pst += d;
-  const double rel = P::div(d, pst + smallpp);
+  const double rel = P::divn(d, pst + smallpp);
   const double arel = (rel > 0) ? rel : -rel;
   return !(arel > 1e-6);
 }
@@ -509,24 +529,29 @@ __device__ __forceinline__ Q4<double> star_flux(double pst, const Q4<double>& ql
   const double half = 0.5, one = 1.0, g6 = ph.gamma6_d;
   const double ps = pst;
   double wr, wl;
-  P::sqrt2(cr * (one + P::q(g6 * (ps - pr), rpr)), cl * (one + P::q(g6 * (ps - pl), rpl)), wr,
+  // Numerators that are never -0 (qn / divn): differences p - ps and ps - p
+  // (p > 0), gamma*po, wo, the fabs'd gamma*ps / rstar, and spout + spin
+  // (neither term is -0: co, cstar >= smallc > 0 and ushock = wo/ro - sg*uo
+  // with wo/ro > 0).  rstar's outer quotient keeps the fix-up (its
+  // denominator is not bounded away from 0).
+  P::sqrt2(cr * (one + P::qn(g6 * (ps - pr), rpr)), cl * (one + P::qn(g6 * (ps - pl), rpl)), wr,
            wl);
   // ((ul + (pl-ps)/wl) + ur) - (pr-ps)/wr, the reference's left-to-right sum
-  const double ustar = half * (((ul + P::div(pl - ps, wl)) + ur) - P::div(pr - ps, wr));
+  const double ustar = half * (((ul + P::divn(pl - ps, wl)) + ur) - P::divn(pr - ps, wr));
   const bool left = ustar > 0.0;
   const double sg = left ? 1.0 : -1.0;
   const double ro = left ? rl : rr;
   const double uo = left ? ul : ur;
   const double po = left ? pl : pr;
   const double wo = left ? wl : wr;
-  double rstar = P::div(ro, one + P::div(ro * (po - ps), wo * wo));
+  double rstar = P::div(ro, one + P::divn(ro * (po - ps), wo * wo));
   rstar = fmax(rstar, ph.smallr);
   const auto rro = P::rcp(ro);
   double co, cstar;  // (riemann.c computes co before rstar; neither depends on the other)
   if constexpr (std::is_same_v<P, FastDiv>) {
     // both arguments bounded; cstar's is +0 when ps is (sqrt(+0) = +0)
-    const double xs = fabs(P::div(ph.gamma * ps, rstar));
-    co = sqrt_core(fabs(P::q(ph.gamma * po, rro)));
+    const double xs = fabs(P::divn(ph.gamma * ps, rstar));
+    co = sqrt_core(fabs(P::qn(ph.gamma * po, rro)));
     cstar = xs == 0.0 ? xs : sqrt_core(xs);
   } else {
     sqrt_fast2(fabs(P::q(ph.gamma * po, rro)), fabs(P::div(ph.gamma * ps, rstar)), co, cstar);
@@ -535,13 +560,13 @@ __device__ __forceinline__ Q4<double> star_flux(double pst, const Q4<double>& ql
   cstar = fmax(ph.smallc, cstar);
   double spout = co - sg * uo;
   double spin = cstar - sg * ustar;
-  const double ushock = P::q(wo, rro) - sg * uo;
+  const double ushock = P::qn(wo, rro) - sg * uo;
   if (ps >= po) {
     spin = ushock;
     spout = ushock;
   }
   const double scr = fmax(spout - spin, ph.smallc + fabs(spout + spin));
-  double frac = (one + P::div(spout + spin, scr)) * half;
+  double frac = (one + P::divn(spout + spin, scr)) * half;
   frac = fmax(0.0, fmin(one, frac));
   double gr, gu, gp;
   if (spout < 0.0) {
@@ -584,7 +609,8 @@ __device__ __forceinline__ Q4<double> riemann_d(const Q4<double>& ql, const Q4<d
   Rcp rpl{pl, 0.0}, rpr{pr, 0.0};
   if (G && ff) {
     FastDiv::sqrt2(cl, cr, wl0, wr0);
-    pst = fmax(FastDiv::div((wr0 * pl + wl0 * pr) + wl0 * wr0 * (ul - ur), wl0 + wr0), 0.0);
+    // numerator: a positive sum plus a product, never -0
+    pst = fmax(FastDiv::divn((wr0 * pl + wl0 * pr) + wl0 * wr0 * (ul - ur), wl0 + wr0), 0.0);
     rpl = rcp_of(pl);
     rpr = rcp_of(pr);
   } else {
