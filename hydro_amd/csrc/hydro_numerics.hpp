@@ -309,8 +309,10 @@ __device__ __forceinline__ void nsqrt2<float>(double x, double y, double& sx, do
 // constoprim.c:62-78 + equation_of_state.c:64-81 for one cell.  U is in sweep
 // slot order (slot 1 = normal momentum).  FAST (FP64, for cells that passed
 // cell_in): FastDiv, the four divisions by rho sharing one reciprocal; the
-// sound-speed argument gamma*p/rho is then in [2^-183, 2^163].
-template <typename R, bool FAST = false>
+// sound-speed argument gamma*p/rho is then in [2^-183, 2^163].  UABS: the
+// caller uses u and v only through |u|, |v| and their squares, so their
+// quotients may lose the sign of a zero (qn).
+template <typename R, bool FAST = false, bool UABS = false>
 __device__ __forceinline__ Prim<R> prim_cell(R U0, R U1, R U2, R U3, const Phys<R>& ph) {
   Prim<R> o;
   o.rinv = (R) 0;
@@ -321,10 +323,17 @@ __device__ __forceinline__ Prim<R> prim_cell(R U0, R U1, R U2, R U3, const Phys<
     using P = std::conditional_t<FAST, FastDiv, IeeeDiv>;
     const auto rr = P::rcp(r);
     if constexpr (FAST) o.rinv = rr.r;
-    u = P::q(U1, rr);
-    v = P::q(U2, rr);
+    if constexpr (UABS) {
+      u = P::qn(U1, rr);
+      v = P::qn(U2, rr);
+    } else {
+      u = P::q(U1, rr);
+      v = P::q(U2, rr);
+    }
     R eken = half * (u * u + v * v);
-    e = P::q(U3, rr) - eken;
+    // (a +-0 quotient, whose sign qn may lose, meets eken: if eken = +0, e is
+    // +-0 and p = max(gm1*r*e, r*smallp) = r*smallp for either sign)
+    e = P::qn(U3, rr) - eken;
     p = ph.gm1 * r * e;
     p = maxm(p, r * ph.smallp);
     o.c = P::sqrt1(P::qn(ph.gamma * p, rr));  // gamma*p > 0
@@ -362,7 +371,7 @@ template <typename R, bool FD = false>
 __device__ __forceinline__ void courant_cell(R U0, R U1, R U2, R U3, const Phys<R>& ph, R& cx,
                                              R& cy) {
   const Prim<R> o = FD && wave_cell_in<R>(U0, U1, U2, U3, ph)
-                        ? prim_cell<R, true>(U0, U1, U2, U3, ph)
+                        ? prim_cell<R, true, true>(U0, U1, U2, U3, ph)
                         : prim_cell<R, false>(U0, U1, U2, U3, ph);
   cx = o.c + fabs(o.q.u);
   cy = o.c + fabs(o.q.v);

@@ -19,6 +19,18 @@ def to_oracle(p: hydro_amd.Params, testcase: int = 0) -> oracle.Params:
     return oracle.Params(**d)
 
 
+def bits_differ(a: np.ndarray, b: np.ndarray) -> int:
+    """Number of values whose bit patterns differ, the sign of a zero included;
+    a NaN matches a NaN whatever its payload."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    assert a.shape == b.shape and a.dtype == b.dtype, (a.shape, b.shape, a.dtype, b.dtype)
+    ut = {8: np.uint64, 4: np.uint32}[a.dtype.itemsize]
+    diff = a.view(ut) != b.view(ut)
+    diff &= ~(np.isnan(a) & np.isnan(b))
+    return int(np.count_nonzero(diff))
+
+
 def rel_l1(a: np.ndarray, b: np.ndarray) -> float:
     """Relative L1 difference of the interiors of two [4, nyt, nxt] states, per field max."""
     a = a[:, 2:-2, 2:-2]

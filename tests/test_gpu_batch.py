@@ -9,7 +9,7 @@ import pytest
 
 import hydro_amd
 import oracle
-from helpers import to_oracle
+from helpers import bits_differ, to_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -39,6 +39,8 @@ def alone(p, bx, by, e, steps):
 def check_same(u, ref, what):
     d = np.argwhere(u != ref)
     assert len(d) == 0, f"{what}: {len(d)} values differ, first {tuple(d[0])}"
+    nb = bits_differ(u, ref)
+    assert nb == 0, f"{what}: {nb} values differ in their bits (signed zeros)"
 
 
 def test_batch_matches_single_grids():

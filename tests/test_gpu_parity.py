@@ -11,7 +11,7 @@ import pytest
 
 import hydro_amd
 import oracle
-from helpers import CORNER_250x125, rel_l1, to_oracle
+from helpers import CORNER_250x125, bits_differ, rel_l1, to_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -31,6 +31,8 @@ def assert_same(gpu, ref, what):
         if len(diff) else "")
     assert l1 <= TOL_L1, msg
     assert len(diff) == 0, msg
+    nb = bits_differ(gpu, ref)
+    assert nb == 0, f"{what}: {nb} values differ in their bits (signed zeros)"
 
 
 def surface_loop(g, steps):

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import hydro_amd
+from helpers import bits_differ
 from hydro_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -39,7 +40,7 @@ def test_results_same_alone_and_in_the_mix(precision):
             assert st == s.DONE and t_sub <= t_adm <= t_done
             u_ref, (t_ref, n_ref, _) = alone(run)
             assert (t_sim, nstep) == (t_ref, n_ref)
-            assert np.array_equal(s.result(rid), u_ref), f"request {rid}"
+            assert bits_differ(s.result(rid), u_ref) == 0, f"request {rid}"
 
 
 def test_results_through_pinned_arena():
@@ -55,14 +56,14 @@ def test_results_through_pinned_arena():
         while s.pump() > 0:
             pass
         for rid, run in zip(ids, runs):
-            assert np.array_equal(s.result(rid), alone(run)[0]), f"request {rid}"
+            assert bits_differ(s.result(rid), alone(run)[0]) == 0, f"request {rid}"
         for rid in ids:
             s.release(rid)
         again = [s.submit(r.params, r.bx, r.by, r.e) for r in runs[:3]]
         while s.pump() > 0:
             pass
         for rid, run in zip(again, runs[:3]):
-            assert np.array_equal(s.result(rid), alone(run)[0]), f"request {rid} (reused)"
+            assert bits_differ(s.result(rid), alone(run)[0]) == 0, f"request {rid} (reused)"
 
 
 def test_budget_and_oversized_request():
@@ -95,7 +96,7 @@ def test_failed_run_is_isolated():
         assert s.info(bad)[0] == s.FAILED
         for rid, run in zip(good, runs):
             assert s.info(rid)[0] == s.DONE
-            assert np.array_equal(s.result(rid), alone(run)[0])
+            assert bits_differ(s.result(rid), alone(run)[0]) == 0
 
 
 def test_mixed_precision_error_vs_fp64():
