@@ -104,8 +104,10 @@ __global__ __launch_bounds__(256) void k_batch_sweep_y(const InstDev<S>* __restr
     double cx = 0.0, cy = 0.0;
     if (j0 < d.g.ny + 2) {  // wave-uniform
       const int j1 = min(j0 + d.yseg, d.g.ny + 2);
+      extern __shared__ __attribute__((aligned(16))) unsigned char ylds[];
+      YStash<S>* ys = reinterpret_cast<YStash<S>*>(ylds) + wave;
       y_march<S, ST1, false, sizeof(S) == 8>(d.g, d.ph, tr, dtdx, in, out, colok, i, j0, j1,
-                                             cx, cy);
+                                             ys, cx, cy);
     }
   }
 }
@@ -237,7 +239,8 @@ int batch_sweep_occupancy(int kind, int waves) {
   const hipError_t e =
       kind == 1
           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_sweep_x<S, true>, 64 * waves, 0)
-          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_sweep_y<S, true>, 64 * waves, 0);
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_sweep_y<S, true>, 64 * waves,
+                                                         waves * sizeof(YStash<S>));
   if (e != hipSuccess) return 2;
   return nb > 0 ? nb : 1;
 }
@@ -257,8 +260,8 @@ static void launch_sweep_part(int kind, const InstDev<S>* inst, const BTask* tas
     hipLaunchKernelGGL((k_batch_sweep_x<S, ST1>), dim3(nb), dim3(64 * tn.waves), 0, st, inst,
                        tasks, ntask, phase, tn.xiter, q);
   else
-    hipLaunchKernelGGL((k_batch_sweep_y<S, ST1>), dim3(nb), dim3(64 * tn.waves), 0, st, inst,
-                       tasks, ntask, phase, tn.yseg, q);
+    hipLaunchKernelGGL((k_batch_sweep_y<S, ST1>), dim3(nb), dim3(64 * tn.waves),
+                       tn.waves * sizeof(YStash<S>), st, inst, tasks, ntask, phase, tn.yseg, q);
 }
 
 template <typename S>

@@ -91,7 +91,9 @@ __global__ __launch_bounds__(256) void k_sweep_y(GridGeom g, Phys<S> ph, double 
   double cx = 0.0, cy = 0.0;
   if (j0 < g.ny + 2) {  // wave-uniform
     const int j1 = min(j0 + yseg, g.ny + 2);  // rows [j0, j1) are updated
-    y_march<S, ST1, COUR, FD>(g, ph, tk, dtdx, in, out, colok, i, j0, j1, cx, cy);
+    extern __shared__ __attribute__((aligned(16))) unsigned char ylds[];
+    YStash<S>* ys = reinterpret_cast<YStash<S>*>(ylds) + wave_id();
+    y_march<S, ST1, COUR, FD>(g, ph, tk, dtdx, in, out, colok, i, j0, j1, ys, cx, cy);
   }
   if (COUR) commit_courant(cx, cy, cour, kCourSlots, blockIdx.x % kCourSlots);
 }
@@ -281,7 +283,8 @@ void launch_sweep(int idim, const GridGeom& g, const PhysArgs& pa, GridView<S> i
   do {                                                                                         \
     const int nsg = (int) cdiv(g.ny, (long) yseg * yw);                                        \
     hipLaunchKernelGGL((k_sweep_y<S, A, B, F>), dim3((unsigned) ncolblk * nsg), dim3(64 * yw),  \
-                       0, st, g, ph, pa.dx, dt_val, ctl, in, out, cour, yseg, ncolblk);        \
+                       yw * sizeof(YStash<S>), st, g, ph, pa.dx, dt_val, ctl, in, out, cour,   \
+                       yseg, ncolblk);                                                          \
   } while (0)
     DISPATCH(HY);
 #undef HY

@@ -7,10 +7,10 @@
 //   neighbours come from DPP wave rotations (v_mov_b32_dpp wave_ror/wave_rol),
 //   so slope, trace, Riemann solve, flux and update all stay in registers.
 // * y sweep: one lane owns one column and marches down a segment of rows with
-//   a three-cell register window (primitives, one carried trace state, one
-//   carried flux), two rows per trip, loading each row one row ahead of its
-//   use.  A wave covers 64 adjacent columns, so every load/store is one
-//   contiguous row piece.
+//   a three-cell window (primitives in registers; the carried trace state
+//   and flux in lane-private LDS slots), two rows per trip, loading each row
+//   one row ahead of its use.  A wave covers 64 adjacent columns, so every
+//   load/store is one contiguous row piece.
 // * Both read one state buffer and write the other (ping-pong).
 // * CFL maxima of the written state can be folded into the epilogue: reduced
 //   wave -> workgroup -> one 64-bit atomic max per workgroup into one of a
@@ -299,6 +299,29 @@ struct Cell {
   S u0, u1, u2, u3;  // conserved, sweep slot order
 };
 
+// Lane-private LDS slots of one wave of the y march: the flux through the
+// face above the row being updated and that row's lower-face trace state,
+// carried from one row to the next through LDS rather than registers (the
+// march's registers then fit four waves per SIMD instead of three).  Only
+// the owning lane reads or writes its slots, so no barrier is needed.
+template <typename S>
+struct YStash {
+  S f[4][64];
+  S xm[4][64];
+};
+
+template <typename S>
+__device__ __forceinline__ void stash_put(S (&b)[4][64], int lane, const Q4<S>& v) {
+  b[0][lane] = v.r;
+  b[1][lane] = v.u;
+  b[2][lane] = v.v;
+  b[3][lane] = v.p;
+}
+template <typename S>
+__device__ __forceinline__ Q4<S> stash_get(const S (&b)[4][64], int lane) {
+  return Q4<S>{b[0][lane], b[1][lane], b[2][lane], b[3][lane]};
+}
+
 template <typename S>
 __device__ __forceinline__ Cell<S> load_y(const GridView<S>& in, long o) {
   // gather with the IU/IV swap of conservar.c:106-109
@@ -309,9 +332,10 @@ template <typename S, bool ST1, bool COUR, bool FD>
 __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
                                         const TraceK<S>& tk, double dtdx,
                                         const GridView<S>& in, const GridView<S>& out,
-                                        bool colok, long i, int j0, int j1, double& cx,
-                                        double& cy) {
+                                        bool colok, long i, int j0, int j1, YStash<S>* ys,
+                                        double& cx, double& cy) {
   const long P = g.pitch;
+  const int lane = threadIdx.x & 63;
   const S st = ph.slope_type;
   const bool o1 = ph.iorder == 1;
 
@@ -344,19 +368,24 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
         qxp_b);
   trace(pcc.q, o1 ? z : slope4<S, ST1>(pb.q, pcc.q, pd.q, st), pcc.c, fb && fC && fD, qxm_c,
         qxp_c);
-  Q4<S> fprev = riemann_flux<S, FD>(qxm_b, qxp_c, ph);  // face (j0-1 | j0)
+  stash_put(ys->f, lane, riemann_flux<S, FD>(qxm_b, qxp_c, ph));  // face (j0-1 | j0)
+  stash_put(ys->xm, lane, qxm_c);
 
-  // window: cell C = row j-2 (being updated), D = row j-1; carried: qxm of C,
-  // flux through the face below C
+  // window: cell C = row j-2 (being updated), D = row j-1; carried in LDS:
+  // qxm of C and the flux through the face below C; C's conserved values are
+  // read again (from cache) for its update
   Q4<S> qC = pcc.q, qD = pd.q;
   S cD = pd.c, rD = pd.rinv;
-  Q4<S> qxmC = qxm_c;
-  Cell<S> uC = uc, uD = ud;
   // the next row's loads are issued one row ahead of their use
   Cell<S> uN = load_y(in, (long) (j0 + 2) * P + i);
   // One row of the march: take row j, prefetch row j+1, update row j-2,
   // shift the window.
   auto row = [&](int j) {
+    // (the compiler must read the slots the previous row wrote, not forward
+    // the values in registers)
+    asm volatile("" ::: "memory");
+    const long oc = (long) (j - 2) * P + i;
+    const Cell<S> uC = load_y(in, oc);
     const Cell<S> uE = uN;
     const int jn = j + 1 <= j1 + 1 ? j + 1 : j1 + 1;  // last row: a harmless reload
     uN = load_y(in, (long) jn * P + i);
@@ -368,12 +397,15 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
       if (fC && fD && fE) trace_cell<S, true, true>(qD, dqD, cD, tk, qxmD, qxpD, rD);
       else trace_cell<S, false>(qD, dqD, cD, tk, qxmD, qxpD);
     }
+    const Q4<S> qxmC = stash_get(ys->xm, lane);
+    stash_put(ys->xm, lane, qxmD);
     const Q4<S> fcur = riemann_flux<S, FD>(qxmC, qxpD, ph);  // face (j-2 | j-1)
+    const Q4<S> fprev = stash_get(ys->f, lane);
+    stash_put(ys->f, lane, fcur);
     const double n0 = update1<S>(uC.u0, fprev.r, fcur.r, dtdx);
     const double n1 = update1<S>(uC.u1, fprev.u, fcur.u, dtdx);
     const double n2 = update1<S>(uC.u2, fprev.v, fcur.v, dtdx);
     const double n3 = update1<S>(uC.u3, fprev.p, fcur.p, dtdx);
-    const long oc = (long) (j - 2) * P + i;
     if (colok) {
       // scatter back with the swap undone (conservar.c:177-180)
       out.p[0][oc] = (S) n0;
@@ -387,16 +419,12 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
         cy = maxm(cy, (double) ay);
       }
     }
-    fprev = fcur;
-    qxmC = qxmD;
     qC = qD;
     qD = pE.q;
     cD = pE.c;
     rD = pE.rinv;
     fC = fD;
     fD = fE;
-    uC = uD;
-    uD = uE;
   };
   // two rows per trip: the window shifts by two, so its registers map back
   // onto themselves without copies
