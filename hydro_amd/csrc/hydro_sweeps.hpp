@@ -300,14 +300,17 @@ struct Cell {
 };
 
 // Lane-private LDS slots of one wave of the y march: the flux through the
-// face above the row being updated and that row's lower-face trace state,
-// carried from one row to the next through LDS rather than registers (the
-// march's registers then fit four waves per SIMD instead of three).  Only
-// the owning lane reads or writes its slots, so no barrier is needed.
+// face above the row being updated, that row's lower-face trace state, and
+// the conserved values of the rows awaiting their update (slot j % 3 for
+// row j), carried from one row to the next through LDS rather than
+// registers (the march's registers then fit four waves per SIMD instead of
+// three; the conserved values are not read from memory twice).  Only the
+// owning lane reads or writes its slots, so no barrier is needed.
 template <typename S>
 struct YStash {
   S f[4][64];
   S xm[4][64];
+  S u[3][4][64];
 };
 
 template <typename S>
@@ -320,6 +323,18 @@ __device__ __forceinline__ void stash_put(S (&b)[4][64], int lane, const Q4<S>& 
 template <typename S>
 __device__ __forceinline__ Q4<S> stash_get(const S (&b)[4][64], int lane) {
   return Q4<S>{b[0][lane], b[1][lane], b[2][lane], b[3][lane]};
+}
+
+template <typename S>
+__device__ __forceinline__ void stash_put_cell(S (&b)[4][64], int lane, const Cell<S>& c) {
+  b[0][lane] = c.u0;
+  b[1][lane] = c.u1;
+  b[2][lane] = c.u2;
+  b[3][lane] = c.u3;
+}
+template <typename S>
+__device__ __forceinline__ Cell<S> stash_get_cell(const S (&b)[4][64], int lane) {
+  return Cell<S>{b[0][lane], b[1][lane], b[2][lane], b[3][lane]};
 }
 
 template <typename S>
@@ -344,6 +359,9 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
   Cell<S> ub = load_y(in, (long) (j0 - 1) * P + i);
   Cell<S> uc = load_y(in, (long) j0 * P + i);
   Cell<S> ud = load_y(in, (long) (j0 + 1) * P + i);
+  // rows j0 and j0+1 await their updates
+  stash_put_cell(ys->u[j0 % 3], lane, uc);
+  stash_put_cell(ys->u[(j0 + 1) % 3], lane, ud);
   // FastDiv per row when the wave's cells of the row are in range (a trace
   // also needs the rows above and below it in range)
   auto prim = [&](const Cell<S>& u, bool f) {
@@ -372,8 +390,8 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
   stash_put(ys->xm, lane, qxm_c);
 
   // window: cell C = row j-2 (being updated), D = row j-1; carried in LDS:
-  // qxm of C and the flux through the face below C; C's conserved values are
-  // read again (from cache) for its update
+  // qxm of C, the flux through the face below C and the conserved values of
+  // C and D
   Q4<S> qC = pcc.q, qD = pd.q;
   S cD = pd.c, rD = pd.rinv;
   // the next row's loads are issued one row ahead of their use
@@ -385,8 +403,9 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
     // the values in registers)
     asm volatile("" ::: "memory");
     const long oc = (long) (j - 2) * P + i;
-    const Cell<S> uC = load_y(in, oc);
     const Cell<S> uE = uN;
+    const int sj = j % 3;  // wave-uniform
+    stash_put_cell(ys->u[sj], lane, uE);
     const int jn = j + 1 <= j1 + 1 ? j + 1 : j1 + 1;  // last row: a harmless reload
     uN = load_y(in, (long) jn * P + i);
     const bool fE = FD && wave_cell_in<S>(uE.u0, uE.u1, uE.u2, uE.u3, ph);
@@ -402,6 +421,10 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
     const Q4<S> fcur = riemann_flux<S, FD>(qxmC, qxpD, ph);  // face (j-2 | j-1)
     const Q4<S> fprev = stash_get(ys->f, lane);
     stash_put(ys->f, lane, fcur);
+    // (read C's conserved values only now: loaded early they would hold
+    // registers through the Riemann solve)
+    asm volatile("" ::: "memory");
+    const Cell<S> uC = stash_get_cell(ys->u[sj == 2 ? 0 : sj + 1], lane);  // row j-2
     const double n0 = update1<S>(uC.u0, fprev.r, fcur.r, dtdx);
     const double n1 = update1<S>(uC.u1, fprev.u, fcur.u, dtdx);
     const double n2 = update1<S>(uC.u2, fprev.v, fcur.v, dtdx);
