@@ -9,8 +9,8 @@
 // same bits as the reference routine cited beside it.
 //
 // Mixed precision (R = float): storage, primitives, slopes, traces, wave
-// speeds and fluxes are float; the Riemann first guess and Newton pressure
-// iteration run in double (riemann_flux), the update accumulates in double
+// speeds and fluxes are float; the Riemann solver's Newton pressure
+// iteration runs in double (riemann_flux), the update accumulates in double
 // (update1) and the CFL maximum and dt are double in the callers.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -254,23 +254,14 @@ __device__ __forceinline__ void xsqrt2(float a, float b, float& sa, float& sb) {
   sb = __builtin_amdgcn_sqrtf(b);
 }
 
-// FP64 division and square roots of the Riemann first guess and Newton
-// iteration, by the sweep's compute type R.  R = double (the FP64 form): the
-// correctly rounded operations.  R = float (mixed precision): the hardware
-// estimates v_rcp_f64 / v_rsq_f64 refined by Newton-Raphson steps
+// FP64 division and square roots of the mixed form's Newton iteration: the
+// hardware estimates v_rcp_f64 / v_rsq_f64 refined by Newton-Raphson steps
 // (relative error a few ulp of FP64, not correctly rounded).
 __device__ __forceinline__ double div_nr(double a, double b) {
   double y = __builtin_amdgcn_rcp(b);
   y = __builtin_fma(__builtin_fma(-b, y, 1.0), y, y);
   const double q = a * y;
   return __builtin_fma(__builtin_fma(-b, q, a), y, q);
-}
-__device__ __forceinline__ double sqrt_nr(double x) {
-  const double g = __builtin_amdgcn_rsq(x);
-  const double s = x * g;
-  const double r = __builtin_fma(-(g * 0.5), s, 0.5);
-  const double v = __builtin_fma(s, r, s);
-  return x == 0.0 ? x : v;  // rsq(0) = inf
 }
 // 1/b: the v_rcp_f64 estimate refined by two Newton-Raphson steps.
 __device__ __forceinline__ double rcp_nr(double b) {
@@ -288,24 +279,6 @@ __device__ __forceinline__ void sqrt_rsq_nr(double x, double& s, double& is) {
   s = __builtin_fma(s0, r, s0);
   is = __builtin_fma(g, r, g);
 }
-template <typename R>
-__device__ __forceinline__ double ndiv(double a, double b);
-template <>
-__device__ __forceinline__ double ndiv<double>(double a, double b) { return a / b; }
-template <>
-__device__ __forceinline__ double ndiv<float>(double a, double b) { return div_nr(a, b); }
-template <typename R>
-__device__ __forceinline__ void nsqrt2(double x, double y, double& sx, double& sy);
-template <>
-__device__ __forceinline__ void nsqrt2<double>(double x, double y, double& sx, double& sy) {
-  sqrt_fast2(x, y, sx, sy);
-}
-template <>
-__device__ __forceinline__ void nsqrt2<float>(double x, double y, double& sx, double& sy) {
-  sx = sqrt_nr(x);
-  sy = sqrt_nr(y);
-}
-
 // constoprim.c:62-78 + equation_of_state.c:64-81 for one cell.  U is in sweep
 // slot order (slot 1 = normal momentum).  FAST (FP64, for cells that passed
 // cell_in): FastDiv, the four divisions by rho sharing one reciprocal; the
@@ -641,9 +614,9 @@ __device__ __forceinline__ Q4<double> riemann_d(const Q4<double>& ql, const Q4<d
                             IeeeDiv::R{pr}, ph);
 }
 
-// The mixed-precision form (R = float) of riemann_flux: FP32 inputs and star
-// state, first guess and Newton iteration in FP64 with the refined hardware
-// estimates (div_nr, sqrt_nr).
+// The mixed-precision form (R = float) of riemann_flux: FP32 inputs, first
+// guess and star state, the Newton iteration in FP64 with the refined
+// hardware estimates (div_nr, rcp_nr, sqrt_rsq_nr).
 template <typename R>
 __device__ __forceinline__ Q4<R> riemann_flux_mixed(const Q4<R>& ql, const Q4<R>& qr,
                                                     const Phys<R>& ph) {
@@ -665,10 +638,14 @@ __device__ __forceinline__ Q4<R> riemann_flux_mixed(const Q4<R>& ql, const Q4<R>
   // bl = wl^2 + cl), qr ql / (qr + ql) = al ar / (al br + ar bl) is one
   // division; |d / (p* + smallpp)| > 1e-6 is tested as |d| > 1e-6 (p* +
   // smallpp) (p* >= 0, smallpp > 0).
+  // The first guess (riemann.c:145-150) is the iteration's starting point
+  // and is formed in FP32: a Newton step squares its relative error (1e-7 ->
+  // 1e-14), so the FP64 iterate it reaches is as accurate as from an FP64
+  // guess.
   const double dpl = pl, dpr = pr, dcl = cl, dcr = cr, dul = ul, dur = ur;
-  double wl0, wr0;
-  nsqrt2<R>(dcl, dcr, wl0, wr0);
-  double pst = fmax(ndiv<R>((wr0 * dpl + wl0 * dpr) + wl0 * wr0 * (dul - dur), wl0 + wr0), 0.0);
+  R wl0, wr0;
+  xsqrt2(cl, cr, wl0, wr0);
+  double pst = fmax((double) xdiv((wr0 * pl + wl0 * pr) + wl0 * wr0 * (ul - ur), wl0 + wr0), 0.0);
   const double g6 = ph.gamma6_d;
   const double gpl = g6 * rcp_nr(dpl), gpr = g6 * rcp_nr(dpr);
   for (int it = 0; it < ph.niter; it++) {
