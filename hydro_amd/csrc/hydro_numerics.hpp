@@ -229,16 +229,12 @@ __device__ __forceinline__ bool cell_in(S U0, S U1, S U2, S U3, const Phys<S>& p
 // correctly rounded multi-instruction sequences.
 __device__ __forceinline__ double xdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float xdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-// c + a/b, c - a/b and a/b - c: in FP64 exactly as written (the quotient
-// rounded, then the sum); in FP32 the quotient a * rcp(b) fuses into the sum
-// (one v_fma_f32).
+// c + a/b and a/b - c: in FP64 exactly as written (the quotient rounded,
+// then the sum); in FP32 the quotient a * rcp(b) fuses into the sum (one
+// v_fma_f32).
 __device__ __forceinline__ double add_div(double c, double a, double b) { return c + a / b; }
 __device__ __forceinline__ float add_div(float c, float a, float b) {
   return __builtin_fmaf(a, __builtin_amdgcn_rcpf(b), c);
-}
-__device__ __forceinline__ double sub_div(double c, double a, double b) { return c - a / b; }
-__device__ __forceinline__ float sub_div(float c, float a, float b) {
-  return __builtin_fmaf(-a, __builtin_amdgcn_rcpf(b), c);
 }
 __device__ __forceinline__ double div_sub(double a, double b, double c) { return a / b - c; }
 __device__ __forceinline__ float div_sub(float a, float b, float c) {
