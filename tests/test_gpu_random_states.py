@@ -69,3 +69,38 @@ def test_random_flows_bitwise(form, seed, slope_type, bounds):
     assert np.isfinite(ur).all()
     nb = bits_differ(u, ur)
     assert nb == 0, f"{nb} values differ in their bits"
+
+
+@pytest.mark.parametrize("form", ["grid", "batch"])
+@pytest.mark.parametrize("gamma,smallr,smallc", [(1.01, 2.0 ** -100, 2.0 ** -50),
+                                                 (3.0, 1e-10, 1e-10),
+                                                 (1.67, 0.5, 2.0)])
+def test_random_flows_at_parameter_bounds(form, gamma, smallr, smallc):
+    """The gated divisions are enabled for gamma in [1.01, 3], smallr in
+    [2^-100, 2^30] and smallc in [2^-50, 2^30]; these runs sit on those
+    bounds (and on floors large enough to bind in many cells: smallr = 0.5
+    against densities from 0.1, smallc = 2 against sound speeds of a few)."""
+    p = hydro_amd.Params(nx=512, ny=480, dx=1.0 / 512, courant_factor=0.8, tend=1e30,
+                         gamma=gamma, smallr=smallr, smallc=smallc)
+    u0 = random_state(p, 50)
+    steps = 12
+    if form == "grid":
+        with hydro_amd.Grid(p) as g:
+            g.upload(u0)
+            dts = g.run(steps)
+            u = g.download()
+    else:
+        with hydro_amd.Batch(hydro_amd.FP64) as b:
+            i = b.add(p)
+            b.upload(i, u0)
+            b.run(steps)
+            u = b.download(i)
+            t, n, _, _ = b.query()[i]
+    ur = u0.copy()
+    rdts, state = oracle.Port().run(to_oracle(p), ur, steps)
+    if form == "grid":
+        assert np.array_equal(dts, rdts), (dts, rdts)
+    else:
+        assert (t, n) == (state[0], state[1])
+    nb = bits_differ(u, ur)
+    assert nb == 0, f"{nb} values differ in their bits"
