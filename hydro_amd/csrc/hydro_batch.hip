@@ -195,7 +195,9 @@ __global__ void k_batch_ctl_end(const InstDev<S>* inst, const int* ids, int nids
   c->t += c->dt;
 }
 
-static inline unsigned cdivu(long a, long b) { return (unsigned) ((a + b - 1) / b); }
+[[maybe_unused]] static inline unsigned cdivu(long a, long b) {
+  return (unsigned) ((a + b - 1) / b);
+}
 
 #ifndef HYDRO_MIXED_UNIT
 __global__ void k_batch_set_ctl(RunCtl* c, double t, double dt, double tend, int nstep,
