@@ -743,10 +743,16 @@ __device__ __forceinline__ Q4<R> riemann_flux(const Q4<R>& ql, const Q4<R>& qr,
 
 
 // conservar.c:147-197: U_i + (F_{i-1/2} - F_{i+1/2}) * dtdx, accumulated in
-// double (exactly the reference expression when R = double).
+// double (exactly the reference expression when R = double).  The fluxes
+// come in double: a flux is converted once (exactly) and serves both cells
+// of its face.
 template <typename R>
-__device__ __forceinline__ double update1(R u, R fl, R fr, double dtdx) {
-  return (double) u + ((double) fl - (double) fr) * dtdx;
+__device__ __forceinline__ double update1(R u, double fl, double fr, double dtdx) {
+  return (double) u + (fl - fr) * dtdx;
+}
+template <typename R>
+__device__ __forceinline__ Q4<double> to_d4(const Q4<R>& v) {
+  return Q4<double>{(double) v.r, (double) v.u, (double) v.v, (double) v.p};
 }
 
 }  // namespace hydro

@@ -162,8 +162,8 @@ __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
   else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);  // (fc is false in FP32)
   const Q4<S> qr = from_right4(qxp);
   // flux through the face between this lane's cell and the next one
-  const Q4<S> fr = riemann_flux<S, FD>(qxm, qr, ph);
-  const Q4<S> fl = from_left4(fr);
+  const Q4<double> fr = to_d4(riemann_flux<S, FD>(qxm, qr, ph));
+  const Q4<double> fl = from_left4(fr);
 
   const bool wr = lane >= 2 && lane <= kXSeg + 1 && i <= g.nx + 1;
   const double n0 = update1<S>(U0, fl.r, fr.r, dtdx);
@@ -192,11 +192,12 @@ __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
 // threads read plausible states (their results are never stored).
 template <typename S, int W>
 struct XLds {
-  S q[4][64 * W + 2], x[4][64 * W + 2], f[4][64 * W + 2];
+  S q[4][64 * W + 2], x[4][64 * W + 2];
+  double f[4][64 * W + 2];  // fluxes, converted once for the update
   int in[W];  // per wave: all its cells passed cell_in
 };
 
-template <typename S, int W>
+template <int W, typename S>
 __device__ __forceinline__ void lds_put(S (&b)[4][64 * W + 2], int t, const Q4<S>& v) {
   b[0][t + 1] = v.r;
   b[1][t + 1] = v.u;
@@ -210,7 +211,7 @@ __device__ __forceinline__ void lds_put(S (&b)[4][64 * W + 2], int t, const Q4<S
     b[3][e] = v.p;
   }
 }
-template <typename S, int W>
+template <int W, typename S>
 __device__ __forceinline__ Q4<S> lds_get(const S (&b)[4][64 * W + 2], int slot) {
   return Q4<S>{b[0][slot], b[1][slot], b[2][slot], b[3][slot]};
 }
@@ -244,7 +245,7 @@ __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
   Q4<S> dq = {(S) 0, (S) 0, (S) 0, (S) 0};
   bool ft = fc;
   if (ph.iorder != 1) {  // launch-uniform
-    lds_put<S, W>(sh.q, t, pc.q);
+    lds_put<W>(sh.q, t, pc.q);
     if constexpr (FD) {
       if ((t & 63) == 0) sh.in[t >> 6] = fc;
     }
@@ -254,8 +255,7 @@ __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
       for (int w = 0; w < W; w++) all &= sh.in[w];
       ft = __builtin_amdgcn_readfirstlane(all) != 0;  // the same in every lane
     }
-    dq = slope4<S, ST1>(lds_get<S, W>(sh.q, t), pc.q, lds_get<S, W>(sh.q, t + 2),
-                        ph.slope_type);
+    dq = slope4<S, ST1>(lds_get<W>(sh.q, t), pc.q, lds_get<W>(sh.q, t + 2), ph.slope_type);
   }
   Q4<S> qxm, qxp;
   if constexpr (FD) {
@@ -264,14 +264,14 @@ __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
   } else {
     trace_cell<S>(pc.q, dq, pc.c, tk, qxm, qxp);
   }
-  lds_put<S, W>(sh.x, t, qxp);
+  lds_put<W>(sh.x, t, qxp);
   __syncthreads();
-  const Q4<S> qr = lds_get<S, W>(sh.x, t + 2);
+  const Q4<S> qr = lds_get<W>(sh.x, t + 2);
   // flux through the face between this thread's cell and the next one
-  const Q4<S> fr = riemann_flux<S, FD>(qxm, qr, ph);
-  lds_put<S, W>(sh.f, t, fr);
+  const Q4<double> fr = to_d4(riemann_flux<S, FD>(qxm, qr, ph));
+  lds_put<W>(sh.f, t, fr);
   __syncthreads();
-  const Q4<S> fl = lds_get<S, W>(sh.f, t);
+  const Q4<double> fl = lds_get<W>(sh.f, t);
 
   const bool wr = t >= 2 && t <= N - 3 && i <= g.nx + 1;
   const double n0 = update1<S>(U0, fl.r, fr.r, dtdx);
@@ -308,7 +308,7 @@ struct Cell {
 // owning lane reads or writes its slots, so no barrier is needed.
 template <typename S>
 struct YStash {
-  S f[4][64];
+  double f[4][64];  // converted once for the update
   S xm[4][64];
   S u[3][4][64];
 };
@@ -386,7 +386,7 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
         qxp_b);
   trace(pcc.q, o1 ? z : slope4<S, ST1>(pb.q, pcc.q, pd.q, st), pcc.c, fb && fC && fD, qxm_c,
         qxp_c);
-  stash_put(ys->f, lane, riemann_flux<S, FD>(qxm_b, qxp_c, ph));  // face (j0-1 | j0)
+  stash_put(ys->f, lane, to_d4(riemann_flux<S, FD>(qxm_b, qxp_c, ph)));  // face (j0-1 | j0)
   stash_put(ys->xm, lane, qxm_c);
 
   // window: cell C = row j-2 (being updated), D = row j-1; carried in LDS:
@@ -418,8 +418,8 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
     }
     const Q4<S> qxmC = stash_get(ys->xm, lane);
     stash_put(ys->xm, lane, qxmD);
-    const Q4<S> fcur = riemann_flux<S, FD>(qxmC, qxpD, ph);  // face (j-2 | j-1)
-    const Q4<S> fprev = stash_get(ys->f, lane);
+    const Q4<double> fcur = to_d4(riemann_flux<S, FD>(qxmC, qxpD, ph));  // face (j-2 | j-1)
+    const Q4<double> fprev = stash_get(ys->f, lane);
     stash_put(ys->f, lane, fcur);
     // (read C's conserved values only now: loaded early they would hold
     // registers through the Riemann solve)
