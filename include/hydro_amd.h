@@ -92,7 +92,8 @@ int hydro_amd_grid_set_limits(hydro_amd_grid *g, double tend, int nstepmax);
  * (1..4) and row pieces each x-sweep workgroup walks.  With 2 or 4 x waves
  * the workgroup shares one row piece of 64*xwaves cells through LDS; with 1
  * or 3 each wave takes its own 64-cell piece and exchanges neighbours by DPP
- * lane rotations. */
+ * lane rotations.  A new grid starts with (64, 1, 4, 8) in FP64 and
+ * (64, 4, 4, 16) in mixed precision. */
 int hydro_amd_grid_set_tune(hydro_amd_grid *g, int yseg, int ywaves, int xwaves, int xiter);
 /* Autotuner: times the candidate launch shapes of both sweeps on this grid
  * (reps launches each, HIP events; the sweeps read the state and write only
