@@ -15,7 +15,10 @@
 // * CFL maxima of the written state can be folded into the epilogue: reduced
 //   wave -> workgroup -> one 64-bit atomic max per workgroup into one of a
 //   set of slots; the bit pattern of a non-negative double orders like its
-//   value, so the integer max is the exact floating max.
+//   value, so the integer max is the exact floating max.  (A NaN Courant
+//   number orders above +inf and takes its slot; the reference's sequential
+//   MAX is order-dependent there, so dt on states holding NaN has no single
+//   reference value.)
 #pragma once
 #include "hydro_kernels.hpp"
 #include "hydro_numerics.hpp"

@@ -335,7 +335,11 @@ void ho_make_boundary(int idim, const ho_params *p, double *uold) {
 double ho_compute_deltat(const ho_params *p, const double *uold) {
   /* HydroC99_2DMpi/Src/compute_deltat.c:55-95 (primitives of interior cells),
    * equation_of_state.c:64-81, courantOnXY :100-145 and the dt formula :197.
-   * The maxima are exact, so the per-row partial maxima combine in any order. */
+   * For finite Courant numbers the maxima are exact, so the per-row partial
+   * maxima combine in any order.  Once the state holds a NaN, MAXM (x > y ?
+   * x : y, the reference's MAX) drops the running maximum at the NaN, and
+   * this result -- like the reference's -- depends on the reduction order,
+   * i.e. on the OpenMP thread count; parity is checked on finite runs. */
   const int nx = p->nx, ny = p->ny;
   const double smallp = SQ(p->smallc) / p->gamma;
   double cx = 0.0, cy = 0.0;
