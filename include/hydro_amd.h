@@ -25,7 +25,12 @@ extern "C" {
 
 #define HYDRO_AMD_FP64 0   /* FP64 storage and arithmetic (bitwise the reference's) */
 #define HYDRO_AMD_MIXED 1  /* FP32 storage / slopes / traces / fluxes; FP64 Newton,
-                              update accumulation and CFL reduction */
+                              update accumulation and CFL reduction.  The total
+                              energy is stored in FP32, so a cell whose internal
+                              energy is below ~1e-5 of its kinetic energy
+                              (Mach ~ 1e2-1e3) loses its pressure to cancellation
+                              and the run can go non-finite: use FP64 for such
+                              flows. */
 
 /* Run parameters: the physics/scheme fields of hydroparam_t
  * (HydroC99_2DMpi/Src/parametres.h:55-95) plus precision. */
