@@ -70,7 +70,7 @@ struct hydro_amd_batch {
   bool dirty = true;
   bool flip = false;             // next step runs Q before P
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  BatchTune tune = {2, 8, 32};  // what the autotuner picks for the batched and serving mixes
+  BatchTune tune = {2, 8, 32};  // set by precision at creation
   int nblk_x = 1024, nblk_y = 1024, cus = 256;
   std::vector<int> live_ids;
   RunCtl* hctl = nullptr;        // pinned host copy of the run controls
@@ -120,6 +120,9 @@ extern "C" hydro_amd_batch* hydro_amd_batch_create(int precision, long long budg
   hydro_amd_batch* b = new hydro_amd_batch();
   b->prec = precision;
   b->esz = precision == HYDRO_AMD_FP64 ? 8 : 4;
+  // what the autotuner picks for the 512-instance and serving mixes (the
+  // serving loop runs with this shape): FP64 (2, 8, 32), mixed (4, 8, 16)
+  b->tune = precision == HYDRO_AMD_FP64 ? BatchTune{2, 8, 32} : BatchTune{4, 8, 16};
   b->budget = budget_bytes > 0 ? budget_bytes : (1LL << 62);
   b->inst.resize(kMaxInst);
   const size_t isz = precision == HYDRO_AMD_FP64 ? sizeof(InstDev<double>) : sizeof(InstDev<float>);
