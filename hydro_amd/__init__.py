@@ -197,6 +197,8 @@ def lib() -> C.CDLL:
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
+    L.hydro_amd_compute_deltat_noalloc.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.hydro_amd_compute_deltat_noalloc.restype = C.c_int
     L.process_args.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(HydroParam)]
     L.process_args.restype = None
     L.vtkfile.argtypes = [C.c_int, HydroParam, C.POINTER(HydroVar)]
@@ -438,6 +440,15 @@ def copy_probe(nbytes: int, reps: int = 10) -> float:
     if v < 0:
         raise HydroError(last_error())
     return float(v)
+
+
+def compute_deltat_noalloc(p: Params, d_uold: int, d_dt: int, stream: int = 0) -> None:
+    """hydro_amd_compute_deltat_noalloc: dt of the FP64 state at device address
+    d_uold ([4, ny+4, nx+4] float64) written to the device double at d_dt, queued
+    on `stream` (a hipStream_t handle, 0 = the null stream) without waiting;
+    allocates no device memory."""
+    _check(lib().hydro_amd_compute_deltat_noalloc(C.byref(p.c()), C.c_void_p(d_uold),
+                                                  C.c_void_p(d_dt), C.c_void_p(stream or None)))
 
 
 def now() -> float:

@@ -197,6 +197,21 @@ int hydro_amd_server_reserve_results(hydro_amd_server *s, long long bytes);
  * and to calibrate HBM byte counters. */
 double hydro_amd_copy_probe(long long bytes, int reps);
 
+/* Allocation-free compute_deltat.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same result as
+ * hydro_amd_grid_compute_deltat (CFL max over the interior, then
+ * dt = courant_factor*dx / max(cx, cy, smallc), compute_deltat.c:166-200) for
+ * an FP64 state the caller keeps on the device in the reference layout
+ * [4][ny+4][nx+4] (d_uold); dt is written to the device double d_dt.  The
+ * kernels are queued on `stream` (a hipStream_t; NULL = the null stream) and
+ * the call returns without waiting.  Scratch: a 16 KiB file-scope device
+ * array per GPU (the CFL max-slots), cleared at the start of every call; no
+ * hipMalloc or hipFree, any grid size up to (nx+4)*(ny+4) = 2^40.  Calls that
+ * overlap on one GPU (other streams, other threads) race on that array. */
+int hydro_amd_compute_deltat_noalloc(const hydro_amd_params *p, const double *d_uold,
+                                     double *d_dt, void *stream);
+
 /* Host mirror of the reference surface (hydro_surface.h): refresh Hv->uold
  * from the GPU, or push host edits of Hv->uold to the GPU. */
 int hydro_amd_sync_host(hydrovar_t *Hv);

@@ -41,3 +41,48 @@ def rel_l1(a: np.ndarray, b: np.ndarray) -> float:
         num = np.abs(a[v] - b[v]).sum()
         worst = max(worst, num / den if den > 0 else num)
     return worst
+
+
+class DeviceArray:
+    """Device memory from the HIP runtime the library itself links (for tests
+    of entry points that take device pointers)."""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            import ctypes as C
+            hydro_amd.lib()  # loads libamdhip64.so.7 as the library's dependency
+            h = C.CDLL("libamdhip64.so.7")  # the instance already loaded
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            for f in (h.hipMalloc, h.hipFree, h.hipMemcpy, h.hipDeviceSynchronize):
+                f.restype = C.c_int
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, nbytes: int):
+        import ctypes as C
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        assert self.hip().hipMalloc(C.byref(p), self.nbytes) == 0, "hipMalloc failed"
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        assert self.hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1) == 0  # host->device
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype)
+        assert out.nbytes <= self.nbytes
+        assert self.hip().hipDeviceSynchronize() == 0
+        assert self.hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) == 0  # device->host
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = None
