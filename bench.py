@@ -1,28 +1,35 @@
 #!/usr/bin/env python3
 """Benchmark: HydroC99 split Godunov/MUSCL step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nx 8192] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nx 8192]
+                    [--precision fast|fp64|mixed] [--no-cpu]
 
-Workload: the 8192x8192 centred blast of the FP64 plain form (E = 1/dx^2 in the
-centre cell, rho = 1, E = 1e-5 elsewhere, dx = 1/8192, reflective walls, CFL
-0.8, MUSCL, slope_type 1, 10 Riemann iterations).  A step is one full time
-step of the reference loop (dt on even steps, x+y sweeps), run on the GPU
-with the state resident in HBM; W untimed steps, then K timed steps.
+Workload: the 8192x8192 centred blast (E = 1/dx^2 in the centre cell, rho = 1,
+E = 1e-5 elsewhere, dx = 1/8192, reflective walls, CFL 0.8, MUSCL,
+slope_type 1, 10 Riemann iterations).  A step is one full time step of the
+reference loop (dt on even steps, x+y sweeps), run on the GPU with the state
+resident in HBM; W untimed steps, then K timed steps.  The headline form is
+the FP64 fast form (FP64 storage and arithmetic, within a per-field relative
+L1 of 1e-12 of the reference); the bit-for-bit FP64 form and the mixed form
+are measured beside it on the same grid.
 
 Prints ONE JSON line.  `value` is interior cell-updates per second (the
-reference's MC/s unit x 1e6).  `roofline` is for the dominant sweep kernel:
-algorithmic bytes (64 B per cell per sweep in FP64: read 4 doubles, write 4)
-over its average duration, timed with HIP events recorded on the grid's own
-stream around every launch in the timed region; its `traffic` is the HBM bytes
-the kernel really moves (two rocprofv3 --pmc passes, calibrated on a copy
-kernel).  `valu` is that kernel's VALU work from a third counter pass: VALU
-instructions per cell, FP64 FLOP/s against the 78.6 TFLOP/s vector peak, and
-the share of SIMD cycles issuing VALU (the sweeps are VALU-bound).
+reference's MC/s unit x 1e6) over the timed steps, run as the product runs
+them: replayed HIP graphs, no per-launch instrumentation.  Per-kernel
+durations come from a second, instrumented pass over the same grid (HIP
+events recorded on the grid's own stream around every launch).  `roofline` is
+for the dominant sweep kernel: algorithmic bytes (64 B per cell per sweep in
+FP64: read 4 doubles, write 4) over its average duration; its `traffic` is
+the HBM bytes the kernel really moves (two rocprofv3 --pmc passes,
+calibrated on a copy kernel).  `valu` is that kernel's VALU work from a
+third counter pass.  `c2` is the 23000x23000 corner blast (50 steps).
 `batched` and `serving` time the many-grid workloads (512 batched instances;
-Poisson arrivals at a nominal and a saturating rate, mixed precision, against
-the plain per-grid FP64 loop).  `cpu_baseline` is the reference's OpenMP CPU
-path (HydroC99_2Domp, built by oracle/Makefile into oracle/_ref/hydro_omp) on
-a bounded sample of the same workload.
+256 Poisson arrivals at a nominal and a saturating rate, mixed precision)
+against the plain per-grid FP64 loop, timed cold (each grid created, run and
+freed in turn) and warm (grids created before the clock starts).
+`cpu_baseline` is the reference's OpenMP CPU path (HydroC99_2Domp, built by
+oracle/Makefile into oracle/_ref/hydro_omp) on a bounded sample of the same
+workload.
 """
 from __future__ import annotations
 
@@ -40,10 +47,17 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/s + % HBM roofline; batched solves/s & p99 latency vs plain FP64"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
-BYTES_PER_CELL_SWEEP = {0: 64, 1: 32}  # FP64 / FP32 storage: 4 vars read + 4 written
+BYTES_PER_CELL_SWEEP = {0: 64, 1: 32, 2: 64}  # FP64 / FP32 / FP64: 4 vars read + 4 written
 
 
-def gpu_bench(nx, ny, steps, warmup, precision, autotune=True, testcase=0):
+PRECISIONS = {"fp64": 0, "mixed": 1, "fast": 2}
+FORM_NAMES = {0: "FP64 (bit for bit)", 1: "mixed FP32/FP64", 2: "FP64 fast form (1e-12)"}
+PREC_ARG = {v: k for k, v in PRECISIONS.items()}
+
+
+def gpu_bench(nx, ny, steps, warmup, precision, autotune=True, testcase=0, prof_steps=20):
+    """Times `steps` steps of one grid on the product path (graph replay), then
+    an instrumented pass of prof_steps steps for per-kernel durations."""
     import hydro_amd
     p = hydro_amd.Params(nx=nx, ny=ny, dx=1.0 / nx, courant_factor=0.8, niter_riemann=10,
                          iorder=2, slope_type=1.0, tend=1e30, nstepmax=2**31 - 1,
@@ -54,21 +68,23 @@ def gpu_bench(nx, ny, steps, warmup, precision, autotune=True, testcase=0):
     tune = g.autotune(reps=5) if autotune else None
     if warmup:
         g.run(warmup, log=False)
-    g.set_profile(True)
     g.sync()
     t0 = time.perf_counter()
     done = g.run(steps, log=False)
     t1 = time.perf_counter()
+    assert done == steps, (done, steps)
+    tstate = g.state()
+    g.set_profile(True)
+    g.run(prof_steps, log=False)
     prof = g.profile()
     g.set_profile(False)
-    tstate = g.state()
     g.close()
-    assert done == steps, (done, steps)
     elapsed = t1 - t0
     cells = nx * ny
     kern = max(("x_sweep", "y_sweep"), key=lambda k: prof[k][0])
     ms, n = prof[kern]
-    per_launch_bytes = BYTES_PER_CELL_SWEEP[precision] * cells
+    bpc = BYTES_PER_CELL_SWEEP[precision]
+    per_launch_bytes = bpc * cells
     avg_s = ms / n / 1e3
     achieved = per_launch_bytes / avg_s / 1e9
     return {
@@ -79,17 +95,31 @@ def gpu_bench(nx, ny, steps, warmup, precision, autotune=True, testcase=0):
         "t_end": tstate[0],
         "kernels": {k: {"avg_ms": (v[0] / v[1] if v[1] else None), "launches": v[1]}
                     for k, v in prof.items()},
+        "kernels_note": f"instrumented pass of {prof_steps} steps after the timed region",
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": kern,
-                     "algorithmic_bytes_per_launch": per_launch_bytes},
+                     "algorithmic_bytes_per_launch": per_launch_bytes,
+                     "bytes_per_cell_sweep": bpc},
+        "step_roofline_frac": round(cells * steps * 2 * bpc / elapsed / 1e9 / HBM_PEAK_GBS, 4),
     }
+
+
+def form_summary(r, precision):
+    return {"form": FORM_NAMES[precision], "value": r["value"], "ms_per_step": r["ms_per_step"],
+            "roofline_frac": r["roofline"]["frac"], "step_roofline_frac": r["step_roofline_frac"],
+            "kernels": r["kernels"], "launch_shape": r["tune"]}
 
 
 def batched_bench(n=512, steps=100):
     """Batched workload: n seeded blast instances (sides log-uniform in [64, 1024]),
-    `steps` steps each: the plain per-grid FP64 loop (one Grid per instance,
-    created, run and freed in turn) against one batch (FP64 and mixed)."""
+    `steps` steps each.  The plain per-grid FP64 loop runs one Grid per
+    instance in turn, timed cold (each grid created, initialised, run and
+    freed inside the clock) and warm (every grid created and initialised
+    before the clock starts; only the runs are timed); the batches (FP64,
+    FP64 fast form, mixed) are timed from their first step, with admission,
+    initialisation and the autotuner (3 timed repetitions per shape) before
+    the clock, and that setup reported beside."""
     import hydro_amd
     from hydro_amd import workloads as W
     out = {"instances": n, "steps": steps,
@@ -100,15 +130,30 @@ def batched_bench(n=512, steps=100):
         with hydro_amd.Grid(r.params) as g:
             g.init_blast(r.bx, r.by, r.e)
             g.run(steps, log=False)
-    t_plain = time.perf_counter() - t0
-    out["plain_fp64_per_grid"] = {"seconds": t_plain, "solves_per_s": n / t_plain}
-    for name, prec in (("batched_fp64", hydro_amd.FP64), ("batched_mixed", hydro_amd.MIXED)):
+    t_cold = time.perf_counter() - t0
+    grids = [hydro_amd.Grid(r.params) for r in runs]
+    for g, r in zip(grids, runs):
+        g.init_blast(r.bx, r.by, r.e)
+    for g in grids:
+        g.sync()
+    t0 = time.perf_counter()
+    for g in grids:
+        g.run(steps, log=False)
+    t_warm = time.perf_counter() - t0
+    for g in grids:
+        g.close()
+    del grids
+    out["plain_fp64_per_grid"] = {"seconds": t_warm, "solves_per_s": n / t_warm,
+                                  "cold_seconds": t_cold, "cold_solves_per_s": n / t_cold}
+    for name, prec in (("batched_fp64", hydro_amd.FP64), ("batched_fp64_fast", hydro_amd.FP64_FAST),
+                       ("batched_mixed", hydro_amd.MIXED)):
         runs = W.batch_instances(n, steps=steps, precision=prec)
+        ts = time.perf_counter()
         with hydro_amd.Batch(prec) as b:
             ids = [b.add(r.params) for r in runs]
             for i, r in zip(ids, runs):
                 b.init_blast(i, r.bx, r.by, r.e)
-            tune = b.autotune(reps=1)
+            tune = b.autotune(reps=3)
             b.sync()
             t0 = time.perf_counter()
             b.run(steps)
@@ -116,8 +161,9 @@ def batched_bench(n=512, steps=100):
             q = b.query()
             assert all(v[1] == steps for v in q.values())
         out[name] = {"seconds": t, "solves_per_s": n / t, "cell_updates_per_s":
-                     out["cell_updates"] / t, "tune": tune,
-                     "speedup_vs_plain_fp64": t_plain / t}
+                     out["cell_updates"] / t, "tune": tune, "setup_seconds": t0 - ts,
+                     "speedup_vs_plain_fp64": t_warm / t,
+                     "speedup_vs_plain_fp64_cold": t_cold / t}
     return out
 
 
@@ -147,18 +193,22 @@ def _serve(runs, chunk):
     return srv, ids, info, max(x[3] for x in info) - t_start
 
 
-def serving_bench(n=32, rates=(20.0, 1000.0), chunk=20):
+def serving_bench(n=256, rates=(20.0, 1000.0), chunk=20):
     """Serving workload: n runs (sides log-uniform in [128, 4096], t_end 0.1 or 500
     steps) arriving as a Poisson process; mixed precision served by the
     continuous-batching loop against the plain per-grid FP64 loop (FIFO, one
     run at a time, each run's service time measured) on the same requests.
-    Latency = finished - arrived (results on the host in both).  The first
-    rate is the nominal one; the last is far above either's capacity, so its
-    solves/s is each side's throughput under a full queue."""
+    The plain loop's service time is measured warm (grid created and
+    initialised before its clock starts: run + download) and cold (creation
+    and initialisation included).  Latency = finished - arrived (results on
+    the host in both).  The first rate is the nominal one; the last is far
+    above either's capacity, so its solves/s is each side's throughput under a
+    full queue.  The mixed results' error: relative L1 and L-infinity of all
+    four fields against the FP64 run of the same request."""
     import numpy as np
     import hydro_amd
     from hydro_amd import workloads as W
-    res = {"requests": n, "chunk_steps": chunk, "rates": []}
+    res = {"requests": n, "chunk_steps": chunk, "budget_bytes": 200 * 10**9, "rates": []}
     served = []
     for i, rate in enumerate(rates):
         runs = W.serving_requests(n, rate=rate, precision=hydro_amd.MIXED)
@@ -167,59 +217,104 @@ def serving_bench(n=32, rates=(20.0, 1000.0), chunk=20):
         served.append({"rate_per_s": rate, "runs": runs,
                        "served_mixed": {"solves_per_s": n / span, "seconds": span,
                                         "p50_s": float(np.percentile(lat, 50)),
-                                        "p99_s": float(np.percentile(lat, 99))}})
+                                        "p99_s": float(np.percentile(lat, 99)),
+                                        "max_s": float(lat.max()),
+                                        "fp64_reruns": srv.reruns}})
         if i == 0:
             keep_srv, keep_ids = srv, ids
+            res["cell_updates"] = float(sum(r.params.nx * r.params.ny * x[5]
+                                            for r, x in zip(runs, info)))
         else:
             srv.close()
-    # the plain per-grid FP64 loop: service time of each run alone (create,
-    # run to t_end / nstepmax, download), and the mixed results' error
-    service, errs = [], []
+    # the plain per-grid FP64 loop: service time of each run alone, and the
+    # mixed results' error against it
+    warm, cold, errs = [], [], []
     for rid, r in zip(keep_ids, served[0]["runs"]):
         p64 = hydro_amd.Params(**{**r.params.__dict__, "precision": hydro_amd.FP64})
-        t0 = time.perf_counter()
+        tc = time.perf_counter()
         with hydro_amd.Grid(p64) as g:
             g.init_blast(r.bx, r.by, r.e)
+            g.sync()
+            t0 = time.perf_counter()
             g.run(10**6, log=False)
             u64 = g.download()
-        service.append(time.perf_counter() - t0)
+            t1 = time.perf_counter()
+        warm.append(t1 - t0)
+        cold.append(t1 - tc)
         um = keep_srv.result(rid)
         keep_srv.release(rid)
         e = []
-        for v in (0, 3):
+        for v in range(4):
             a, b = um[v, 2:-2, 2:-2], u64[v, 2:-2, 2:-2]
-            e.append((float(np.abs(a - b).sum() / np.abs(b).sum()),
-                      float(np.abs(a - b).max() / np.abs(b).max())))
+            d = np.abs(a - b)
+            e.append((float(d.sum() / np.abs(b).sum()), float(d.max() / np.abs(b).max())))
         errs.append(e)
         del um, u64
     keep_srv.close()
     for s in served:
-        done, lat_plain = 0.0, []
-        for r, sv in zip(s.pop("runs"), service):  # FIFO queue of one server
-            done = max(done, r.arrival) + sv
-            lat_plain.append(done - r.arrival)
-        s["plain_fp64_per_grid"] = {"solves_per_s": n / done, "seconds": done,
-                                    "p50_s": float(np.percentile(lat_plain, 50)),
-                                    "p99_s": float(np.percentile(lat_plain, 99))}
-        s["speedup_solves_per_s"] = s["served_mixed"]["solves_per_s"] / (n / done)
+        runs = s.pop("runs")
+        for key, service in (("plain_fp64_per_grid", warm), ("plain_fp64_per_grid_cold", cold)):
+            done, lat_plain = 0.0, []
+            for r, sv in zip(runs, service):  # FIFO queue of one server
+                done = max(done, r.arrival) + sv
+                lat_plain.append(done - r.arrival)
+            s[key] = {"solves_per_s": n / done, "seconds": done,
+                      "p50_s": float(np.percentile(lat_plain, 50)),
+                      "p99_s": float(np.percentile(lat_plain, 99))}
+        s["speedup_solves_per_s"] = s["served_mixed"]["solves_per_s"] / s[
+            "plain_fp64_per_grid"]["solves_per_s"]
+        s["speedup_solves_per_s_vs_cold"] = s["served_mixed"]["solves_per_s"] / s[
+            "plain_fp64_per_grid_cold"]["solves_per_s"]
         res["rates"].append(s)
-    errs = np.array(errs)
-    res["plain_fp64_service_s_total"] = float(sum(service))
-    res["mixed_error_vs_fp64"] = {"density_l1_max": float(errs[:, 0, 0].max()),
-                                  "density_linf_max": float(errs[:, 0, 1].max()),
-                                  "energy_l1_max": float(errs[:, 1, 0].max()),
-                                  "energy_linf_max": float(errs[:, 1, 1].max()),
-                                  # per run, in request order: relative L1 and
-                                  # L-infinity of density and of energy
-                                  "per_run": [[float(f"{x:.3g}") for x in e.ravel()]
-                                              for e in errs]}
+    errs = np.array(errs)  # [run, field, (L1, Linf)]
+    names = ("density", "momentum_x", "momentum_y", "energy")
+    res["plain_fp64_service_s_total"] = {"warm": float(sum(warm)), "cold": float(sum(cold))}
+    res["mixed_error_vs_fp64"] = {
+        **{f"{nm}_l1_max": float(errs[:, v, 0].max()) for v, nm in enumerate(names)},
+        **{f"{nm}_linf_max": float(errs[:, v, 1].max()) for v, nm in enumerate(names)},
+        "note": "relative L1 (sum|a-b|/sum|b|) and L-infinity (max|a-b|/max|b|) of the interior "
+                "per field against the FP64 run of the same request, worst over the requests"}
     return res
 
 
 def cpu_threads():
-    n = os.cpu_count() or 1
-    # one GPU's share of the box's host cores
+    """Threads for the CPU baseline: one GPU's share of the box's host cores
+    (16 on these boxes, which allot that share to a one-GPU job: OpenMP pools
+    stay within it), unless HYDRO_CPU_THREADS says otherwise."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
     return max(1, min(n, int(os.environ.get("HYDRO_CPU_THREADS", "16"))))
+
+
+def host_facts():
+    """The host's CPU as lscpu reports it (model, sockets, physical cores,
+    threads per core), the CPUs this process may run on, and the cgroup CPU
+    quota, if any."""
+    out = {}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        kv = dict(l.split(":", 1) for l in txt.splitlines() if ":" in l)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        sockets = int(kv.get("Socket(s)", "0") or 0)
+        per = int(kv.get("Core(s) per socket", "0") or 0)
+        out = {"model": kv.get("Model name"), "sockets": sockets,
+               "physical_cores": sockets * per if sockets and per else None,
+               "threads_per_core": int(kv.get("Thread(s) per core", "0") or 0) or None,
+               "logical_cpus": int(kv.get("CPU(s)", "0") or 0) or None}
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    try:
+        out["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            out["cgroup_cpu_max"] = f.read().strip()
+    except OSError:
+        pass
+    return out
 
 
 def cpu_baseline(nx, ny, nsteps=8, testcase=0):
@@ -253,7 +348,11 @@ def cpu_baseline(nx, ny, nsteps=8, testcase=0):
             tail = per_step[1:] or per_step
             mcs = sum(tail) / max(1, len(tail))
         return {"value": mcs * 1e6, "unit": "cell-updates/s", "cores": threads,
-                "kind": "reference", "sample": sample + " (HydroC99_2Domp, gcc -O3 -fopenmp)"}
+                "kind": "reference", "sample": sample + " (HydroC99_2Domp, gcc -O3 -fopenmp)",
+                "host": host_facts(),
+                "cores_note": "OpenMP threads = this job's share of the host (one GPU's share, "
+                              "the OMP_NUM_THREADS the box sets); the host's whole CPU is in "
+                              "`host`"}
     import oracle
     p = oracle.Params(nx=nx, ny=ny, dx=1.0 / nx, courant_factor=0.8, tend=1e30,
                       testcase=testcase)
@@ -327,7 +426,7 @@ def measured_traffic(nx, precision, tune):
     """HBM bytes per launch of each sweep kernel: rocprofv3 FETCH_SIZE and
     WRITE_SIZE passes over pmc_workload, each scaled by what the same counter
     reads for the copy kernel (COPY_PROBE_BYTES read and written)."""
-    args = ["--steps", "2", "--nx", str(nx), "--precision", ["fp64", "mixed"][precision]]
+    args = ["--steps", "2", "--nx", str(nx), "--precision", PREC_ARG[precision]]
     if tune:
         args += ["--tune", ",".join(str(tune[k]) for k in ("yseg", "ywaves", "xwaves", "xiter"))]
     fetch, write = _pmc_kib("FETCH_SIZE", args), _pmc_kib("WRITE_SIZE", args)
@@ -354,7 +453,7 @@ def measured_valu(nx, precision, tune, kern, avg_s):
     kernel's measured duration `avg_s`, and the share of SIMD cycles issuing
     VALU, counting each wave64 instruction at 4 cycles (a 16-lane SIMD; the
     FP32-heavy mixed form reads above 1, so some instructions take fewer)."""
-    args = ["--steps", "2", "--nx", str(nx), "--precision", ["fp64", "mixed"][precision]]
+    args = ["--steps", "2", "--nx", str(nx), "--precision", PREC_ARG[precision]]
     if tune:
         args += ["--tune", ",".join(str(tune[k]) for k in ("yseg", "ywaves", "xwaves", "xiter"))]
     r = _pmc(VALU_COUNTERS, args)
@@ -384,15 +483,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--nx", type=int, default=8192)
     ap.add_argument("--ny", type=int, default=0)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=sorted(PRECISIONS), default="fast")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--testcase", type=int, default=0)
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the other arithmetic forms on the same grid")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 23000x23000 corner blast")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--batch-n", type=int, default=512)
-    ap.add_argument("--serve-n", type=int, default=32)
+    ap.add_argument("--serve-n", type=int, default=256)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--pmc-workload", action="store_true",
                     help="short fixed run for rocprofv3 counter passes")
@@ -405,13 +507,14 @@ def main():
               "multi-GPU row-band decomposition is a design, not built", file=sys.stderr)
         sys.exit(2)
     ny = a.ny or a.nx
-    prec = 0 if a.precision == "fp64" else 1
+    prec = PRECISIONS[a.precision]
     if a.pmc_workload:
         pmc_workload(a.nx, prec, a.steps,
                      [int(v) for v in a.tune.split(",")] if a.tune else None)
         return
     r = gpu_bench(a.nx, ny, a.steps, a.warmup, prec, autotune=not a.no_autotune,
                   testcase=a.testcase)
+    blast = "centred" if a.testcase == 0 else "corner"
     line = {
         "metric": METRIC,
         "value": r["value"],
@@ -423,17 +526,35 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == 0 else "f32+f64",
+        "dtype": "f32+f64" if prec == 1 else "f64",
         "data": "synthetic",
-        "config": {"workload": f"{a.nx}x{ny} {'centred' if a.testcase == 0 else 'corner'} "
-                               "blast, split Godunov/MUSCL, "
-                               f"{'FP64' if prec == 0 else 'mixed FP32/FP64'}, "
-                               "CFL 0.8, reflective walls",
-                   "nx": a.nx, "ny": ny, "testcase": a.testcase, "steps_timed": a.steps,
-                   "first_timed_step": a.warmup, "launch_shape": r["tune"]},
+        "config": {"workload": f"{a.nx}x{ny} {blast} blast, split Godunov/MUSCL, "
+                               f"{FORM_NAMES[prec]}, CFL 0.8, reflective walls",
+                   "form": FORM_NAMES[prec], "nx": a.nx, "ny": ny, "testcase": a.testcase,
+                   "steps_timed": a.steps, "first_timed_step": a.warmup,
+                   "launch_shape": r["tune"], "timed_path": "replayed HIP graphs"},
         "roofline": r["roofline"],
+        "step_roofline_frac": r["step_roofline_frac"],
         "kernels": r["kernels"],
+        "kernels_note": r["kernels_note"],
     }
+    if not a.no_forms:
+        forms = {a.precision: form_summary(r, prec)}
+        for name, p2 in sorted(PRECISIONS.items()):
+            if p2 != prec:
+                forms[name] = form_summary(
+                    gpu_bench(a.nx, ny, a.steps, a.warmup, p2, autotune=not a.no_autotune,
+                              testcase=a.testcase), p2)
+        line["forms"] = forms
+    if not a.no_c2:
+        c2 = {}
+        for name in (a.precision, "fp64") if a.precision != "fp64" else ("fp64",):
+            p2 = PRECISIONS[name]
+            rc = gpu_bench(23000, 23000, 50, 4, p2, autotune=not a.no_autotune, testcase=1,
+                           prof_steps=6)
+            c2[name] = {**form_summary(rc, p2), "roofline": rc["roofline"]}
+        line["c2"] = {"workload": "23000x23000 corner blast, 50 steps (33.9 GB of FP64 state "
+                                  "in HBM)", **c2}
     if not a.no_traffic:
         tr = measured_traffic(a.nx, prec, r["tune"])
         kern = line["roofline"]["kernel"]

@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libhydro_amd.so")
 
 FP64 = 0
 MIXED = 1
+FP64_FAST = 2  # FP64, not bit for bit: within a per-field relative L1 of 1e-12
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
@@ -186,6 +187,7 @@ def lib() -> C.CDLL:
     L.hydro_amd_server_destroy.argtypes = [S]
     L.hydro_amd_server_destroy.restype = None
     for name, args in [("hydro_amd_server_submit", [S, P, C.c_int, C.c_int, C.c_double]),
+                       ("hydro_amd_server_submit_state", [S, P, _dp]),
                        ("hydro_amd_server_pump", [S]),
                        ("hydro_amd_server_counts", [S, C.c_void_p, C.c_void_p, C.c_void_p,
                                                     C.c_void_p]),
@@ -193,7 +195,9 @@ def lib() -> C.CDLL:
                                                   C.c_void_p]),
                        ("hydro_amd_server_result", [S, C.c_int, _dp]),
                        ("hydro_amd_server_release", [S, C.c_int]),
-                       ("hydro_amd_server_reserve_results", [S, C.c_longlong])]:
+                       ("hydro_amd_server_reserve_results", [S, C.c_longlong]),
+                       ("hydro_amd_server_result_precision", [S, C.c_int]),
+                       ("hydro_amd_server_reruns", [S])]:
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
@@ -491,6 +495,15 @@ class Server:
         self.params[r] = p
         return r
 
+    def submit_state(self, p: Params, u: np.ndarray) -> int:
+        """Queue a run from a whole initial state [4, ny+4, nx+4] (float64)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.shape != p.shape():
+            raise ValueError(f"uold shape {u.shape} != {p.shape()}")
+        r = _check(lib().hydro_amd_server_submit_state(self._h, C.byref(p.c()), u))
+        self.params[r] = p
+        return r
+
     def pump(self) -> int:
         return _check(lib().hydro_amd_server_pump(self._h))
 
@@ -519,6 +532,15 @@ class Server:
 
     def release(self, r: int) -> None:
         _check(lib().hydro_amd_server_release(self._h, r))
+
+    def result_precision(self, r: int) -> int:
+        """The precision request r's result was computed in (FP64 for a mixed
+        run that broke down and was run again)."""
+        return _check(lib().hydro_amd_server_result_precision(self._h, r))
+
+    @property
+    def reruns(self) -> int:
+        return _check(lib().hydro_amd_server_reruns(self._h))
 
     def reserve_results(self, nbytes: int) -> None:
         """Pin host memory for kept results (copied by DMA beside the kernels)."""

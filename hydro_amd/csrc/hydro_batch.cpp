@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -77,9 +78,15 @@ struct hydro_amd_batch {
   void* hidev = nullptr;         // pinned host copy of the instance array
 };
 
+// FP64 storage (the FP64 and FP64 fast forms) rather than FP32.
+static bool dbl(const hydro_amd_batch* b) { return b->prec != HYDRO_AMD_MIXED; }
+
 static void size_grids(hydro_amd_batch* b) {
   const int w = b->tune.waves;
-  if (b->prec == HYDRO_AMD_FP64) {
+  if (b->prec == HYDRO_AMD_FP64_FAST) {
+    b->nblk_x = b->cus * batch_sweep_occupancy_fast(1, w);
+    b->nblk_y = b->cus * batch_sweep_occupancy_fast(2, w);
+  } else if (b->prec == HYDRO_AMD_FP64) {
     b->nblk_x = b->cus * batch_sweep_occupancy<double>(1, w);
     b->nblk_y = b->cus * batch_sweep_occupancy<double>(2, w);
   } else {
@@ -113,19 +120,20 @@ extern "C" void hydro_amd_batch_destroy(hydro_amd_batch* b) {
 }
 
 extern "C" hydro_amd_batch* hydro_amd_batch_create(int precision, long long budget_bytes) {
-  if (precision != HYDRO_AMD_FP64 && precision != HYDRO_AMD_MIXED) {
+  if (precision != HYDRO_AMD_FP64 && precision != HYDRO_AMD_MIXED &&
+      precision != HYDRO_AMD_FP64_FAST) {
     hydro_amd_set_error("unknown precision %d", precision);
     return nullptr;
   }
   hydro_amd_batch* b = new hydro_amd_batch();
   b->prec = precision;
-  b->esz = precision == HYDRO_AMD_FP64 ? 8 : 4;
+  b->esz = dbl(b) ? 8 : 4;
   // what the autotuner picks for the 512-instance and serving mixes (the
   // serving loop runs with this shape): FP64 (2, 8, 32), mixed (4, 8, 16)
-  b->tune = precision == HYDRO_AMD_FP64 ? BatchTune{2, 8, 32} : BatchTune{4, 8, 16};
+  b->tune = precision == HYDRO_AMD_MIXED ? BatchTune{4, 8, 16} : BatchTune{2, 8, 32};
   b->budget = budget_bytes > 0 ? budget_bytes : (1LL << 62);
   b->inst.resize(kMaxInst);
-  const size_t isz = precision == HYDRO_AMD_FP64 ? sizeof(InstDev<double>) : sizeof(InstDev<float>);
+  const size_t isz = dbl(b) ? sizeof(InstDev<double>) : sizeof(InstDev<float>);
   bool ok = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) == hipSuccess &&
             hipMalloc(&b->ctl, sizeof(RunCtl) * kMaxInst) == hipSuccess &&
             hipMalloc(&b->cour, sizeof(uint64_t) * 2 * kInstSlots * kMaxInst) == hipSuccess &&
@@ -254,7 +262,7 @@ extern "C" int hydro_amd_batch_remove(hydro_amd_batch* b, int id) {
 static int init_inst(hydro_amd_batch* b, int id, int bx, int by0, int by1, double e) {
   Inst* in = live(b, id);
   if (!in) return -1;
-  if (b->prec == HYDRO_AMD_FP64)
+  if (dbl(b))
     launch_init<double>(in->g, view_of<double>(b, *in, 0), view_of<double>(b, *in, 1), bx, by0,
                         by1, e, b->st);
   else
@@ -327,6 +335,47 @@ int hydro_amd_batch_download_raw(hydro_amd_batch* b, int id, void* dst, int* ele
   return 0;
 }
 
+static int rebuild(hydro_amd_batch* b);
+
+// Whether each listed live instance's interior state is finite: one CFL pass
+// over it (k_batch_courant) into its slots, whose maxima are NaN or inf
+// exactly when some cell's density, momenta or energy is (a NaN orders above
+// +inf in the slots' integer maximum).  Meant for instances whose run has
+// ended -- it overwrites their CFL slots -- such as the serving loop's
+// finished requests.  Waits for the batch's stream.
+int hydro_amd_batch_finite(hydro_amd_batch* b, const int* ids, int n, int* finite) {
+  if (!b || n < 0 || (n > 0 && (!ids || !finite))) return hydro_amd_set_error("bad argument");
+  if (n == 0) return 0;
+  if (n > kMaxInst) return hydro_amd_set_error("too many instances");
+  BCHK(hipStreamSynchronize(b->st));
+  if (b->dirty && rebuild(b)) return -1;
+  for (int k = 0; k < n; k++) {
+    if (!live(b, ids[k])) return -1;
+    launch_zero_u64(b->cour + (size_t) ids[k] * 2 * kInstSlots, 2 * kInstSlots, b->st);
+  }
+  BCHK(hipMemcpyAsync(b->cids, ids, sizeof(int) * n, hipMemcpyHostToDevice, b->st));
+  if (dbl(b))
+    launch_batch_courant<double>((const InstDev<double>*) b->idev, b->cids, n, b->st);
+  else
+    launch_batch_courant<float>((const InstDev<float>*) b->idev, b->cids, n, b->st);
+  std::vector<uint64_t> slots((size_t) n * 2 * kInstSlots);
+  for (int k = 0; k < n; k++)
+    BCHK(hipMemcpyAsync(slots.data() + (size_t) k * 2 * kInstSlots,
+                        b->cour + (size_t) ids[k] * 2 * kInstSlots,
+                        sizeof(uint64_t) * 2 * kInstSlots, hipMemcpyDeviceToHost, b->st));
+  BCHK(hipStreamSynchronize(b->st));  // (also: `ids` and `slots` outlive the copies)
+  for (int k = 0; k < n; k++) {
+    int ok = 1;
+    for (int q = 0; q < 2 * kInstSlots; q++) {
+      double v;
+      memcpy(&v, &slots[(size_t) k * 2 * kInstSlots + q], sizeof v);
+      if (!std::isfinite(v)) ok = 0;
+    }
+    finite[k] = ok;
+  }
+  return 0;
+}
+
 extern "C" int hydro_amd_batch_upload(hydro_amd_batch* b, int id, const double* u) {
   Inst* in = live(b, id);
   if (!in || !u) return in ? hydro_amd_set_error("null argument") : -1;
@@ -334,7 +383,7 @@ extern "C" int hydro_amd_batch_upload(hydro_amd_batch* b, int id, const double* 
   std::vector<float> tmp;
   const void* srcp = u;
   size_t w = 8;
-  if (b->prec != HYDRO_AMD_FP64) {
+  if (!dbl(b)) {
     tmp.resize((size_t) nxt * rows);
     for (size_t k = 0; k < tmp.size(); k++) tmp[k] = (float) u[k];
     srcp = tmp.data();
@@ -469,7 +518,7 @@ static int rebuild(hydro_amd_batch* b) {
   if (b->nids)
     BCHK(hipMemcpy(b->ids, b->live_ids.data(), sizeof(int) * b->nids, hipMemcpyHostToDevice));
   if (b->maxid >= 0) {
-    if (b->prec == HYDRO_AMD_FP64) {
+    if (dbl(b)) {
       auto* v = (InstDev<double>*) b->hidev;
       for (int id : b->live_ids) v[id] = inst_dev<double>(b, id);
       BCHK(hipMemcpy(b->idev, v, sizeof(v[0]) * (b->maxid + 1), hipMemcpyHostToDevice));
@@ -484,6 +533,22 @@ static int rebuild(hydro_amd_batch* b) {
   return 0;
 }
 
+// The sweeps of task list k (kind 1: x, 2: y) in the batch's arithmetic form,
+// reading buffer `phase`; q: the list's two task-queue counters.
+template <typename S>
+static void sweep_list(hydro_amd_batch* b, int kind, const InstDev<S>* I, int k, int phase,
+                       int nblk, uint64_t* q) {
+  if constexpr (sizeof(S) == 8) {
+    if (b->prec == HYDRO_AMD_FP64_FAST) {
+      launch_batch_sweep_fast(kind, I, b->tasks + b->off[k], b->n[k], b->n1[k], phase, b->tune,
+                              nblk, q, b->st);
+      return;
+    }
+  }
+  launch_batch_sweep<S>(kind, I, b->tasks + b->off[k], b->n[k], b->n1[k], phase, b->tune, nblk,
+                        q, b->st);
+}
+
 // One phase: its ghost fills, then its x tasks and its y tasks.
 template <typename S>
 static void enq_phase(hydro_amd_batch* b, int which, int phase) {
@@ -491,10 +556,8 @@ static void enq_phase(hydro_amd_batch* b, int which, int phase) {
   const int x = which == 0 ? 0 : 2, y = x + 1, bd = which == 0 ? 4 : 5;
   launch_batch_boundary<S>(I, b->tasks + b->off[bd], b->n[bd], phase, b->nblk_y, b->st);
   uint64_t* q = b->qctr + 4 * phase;  // x: q[0..1], y: q[2..3]
-  launch_batch_sweep<S>(1, I, b->tasks + b->off[x], b->n[x], b->n1[x], phase, b->tune, b->nblk_x,
-                        q, b->st);
-  launch_batch_sweep<S>(2, I, b->tasks + b->off[y], b->n[y], b->n1[y], phase, b->tune, b->nblk_y,
-                        q + 2, b->st);
+  sweep_list<S>(b, 1, I, x, phase, b->nblk_x, q);
+  sweep_list<S>(b, 2, I, y, phase, b->nblk_y, q + 2);
 }
 
 template <typename S>
@@ -507,7 +570,7 @@ static void enq_step_t(hydro_amd_batch* b, bool flip) {
 }
 
 static void enq_step(hydro_amd_batch* b, bool flip) {
-  if (b->prec == HYDRO_AMD_FP64) enq_step_t<double>(b, flip);
+  if (dbl(b)) enq_step_t<double>(b, flip);
   else enq_step_t<float>(b, flip);
 }
 
@@ -521,7 +584,7 @@ static int pending_courant(hydro_amd_batch* b) {
   if (need.empty()) return 0;
   BCHK(hipMemcpyAsync(b->cids, need.data(), sizeof(int) * need.size(), hipMemcpyHostToDevice,
                       b->st));
-  if (b->prec == HYDRO_AMD_FP64)
+  if (dbl(b))
     launch_batch_courant<double>((const InstDev<double>*) b->idev, b->cids, (int) need.size(),
                                  b->st);
   else
@@ -659,14 +722,12 @@ extern "C" int hydro_amd_batch_autotune(hydro_amd_batch* b, int reps, int* best)
           for (int k = 0; k < 4; k++) {
             const int kind = (k % 2 == 0) ? 1 : 2;
             const int nb = kind == 1 ? b->nblk_x : b->nblk_y;
-            if (b->prec == HYDRO_AMD_FP64)
-              launch_batch_sweep<double>(kind, (const InstDev<double>*) b->idev,
-                                         b->tasks + b->off[k], b->n[k], b->n1[k], 0, b->tune,
-                                         nb, b->qctr + 2 * k, b->st);
+            if (dbl(b))
+              sweep_list<double>(b, kind, (const InstDev<double>*) b->idev, k, 0, nb,
+                                 b->qctr + 2 * k);
             else
-              launch_batch_sweep<float>(kind, (const InstDev<float>*) b->idev,
-                                        b->tasks + b->off[k], b->n[k], b->n1[k], 0, b->tune, nb,
-                                        b->qctr + 2 * k, b->st);
+              sweep_list<float>(b, kind, (const InstDev<float>*) b->idev, k, 0, nb,
+                                b->qctr + 2 * k);
           }
         }
         (void) hipEventRecord(e1, b->st);

@@ -17,9 +17,11 @@
 // so the CFL reduction of a task never mixes instances.  Run control (t, dt,
 // nstep, active) and CFL slots are per instance, on the device.
 //
-// Compiled twice, as hydro_kernels.hip: the FP64 unit (-ffp-contract=off)
-// and the mixed-precision unit (HYDRO_MIXED_UNIT, -ffp-contract=on) with
-// only the FP32-storage sweeps and CFL kernels.
+// Compiled three times, as hydro_kernels.hip: the FP64 unit
+// (-ffp-contract=off), the mixed-precision unit (HYDRO_MIXED_UNIT,
+// -ffp-contract=on) with only the FP32-storage sweeps and CFL kernels, and
+// the fast unit (HYDRO_FAST_UNIT, -ffp-contract=on) with only the FP64
+// fast-form sweeps.
 #include "hydro_batch.hpp"
 #include "hydro_sweeps.hpp"
 
@@ -42,7 +44,9 @@ __device__ __forceinline__ int next_task(uint64_t* qnext) {
 // x tasks of one phase (every task's kind is 1).  ST1: every listed
 // instance has slope_type == 1 (the host splits each phase's lists), so the
 // limiter's multiply/divide by slope_type is compiled out as in single grids.
-template <typename S, bool ST1>
+// FD: the arithmetic form (persistent workgroups keep the SIMDs busy, so the
+// FP64 batches take the range-gated divisions, kGated, for every instance).
+template <typename S, bool ST1, int FD>
 __global__ __launch_bounds__(256) void k_batch_sweep_x(const InstDev<S>* __restrict__ inst,
                                                        const BTask* __restrict__ tasks,
                                                        int ntask, int phase, int xiter,
@@ -65,14 +69,10 @@ __global__ __launch_bounds__(256) void k_batch_sweep_x(const InstDev<S>* __restr
     // segments tk.a + it * wpb + wave, it = 0 .. xiter-1
     SegWalk w(tk.a + wave, d.nseg);
     for (int it = 0; it < xiter && w.row < d.g.ny; it++) {
-      // persistent workgroups keep the SIMDs busy: the FP64 range-gated
-      // divisions pay for every instance
       if (cour)
-        x_segment<S, ST1, true, sizeof(S) == 8>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx,
-                                                cy);
+        x_segment<S, ST1, true, FD>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx, cy);
       else
-        x_segment<S, ST1, false, sizeof(S) == 8>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx,
-                                                 cy);
+        x_segment<S, ST1, false, FD>(d.g, d.ph, tr, dtdx, in, out, w.row, w.seg, cx, cy);
       w.advance(wpb, d.nseg);
     }
     if (cour) commit_courant(cx, cy, d.cour, kInstSlots, t % kInstSlots);
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void k_batch_sweep_x(const InstDev<S>* __restr
 
 // y tasks of one phase (every task's kind is 2); y sweeps never end a step
 // whose successor needs dt, so they fold no CFL maxima.
-template <typename S, bool ST1>
+template <typename S, bool ST1, int FD>
 __global__ __launch_bounds__(256) void k_batch_sweep_y(const InstDev<S>* __restrict__ inst,
                                                        const BTask* __restrict__ tasks,
                                                        int ntask, int phase, int yseg,
@@ -106,8 +106,7 @@ __global__ __launch_bounds__(256) void k_batch_sweep_y(const InstDev<S>* __restr
       const int j1 = min(j0 + d.yseg, d.g.ny + 2);
       extern __shared__ __attribute__((aligned(16))) unsigned char ylds[];
       YStash<S>* ys = reinterpret_cast<YStash<S>*>(ylds) + wave;
-      y_march<S, ST1, false, sizeof(S) == 8>(d.g, d.ph, tr, dtdx, in, out, colok, i, j0, j1,
-                                             ys, cx, cy);
+      y_march<S, ST1, false, FD>(d.g, d.ph, tr, dtdx, in, out, colok, i, j0, j1, ys, cx, cy);
     }
   }
 }
@@ -199,7 +198,7 @@ __global__ void k_batch_ctl_end(const InstDev<S>* inst, const int* ids, int nids
   return (unsigned) ((a + b - 1) / b);
 }
 
-#ifndef HYDRO_MIXED_UNIT
+#ifdef HYDRO_BASE_UNIT
 __global__ void k_batch_set_ctl(RunCtl* c, double t, double dt, double tend, int nstep,
                                 int nstepmax, int active) {
   if (threadIdx.x != 0) return;
@@ -233,47 +232,82 @@ void launch_zero_u64(uint64_t* p, long n, hipStream_t st) {
   if (nb > 1024) nb = 1024;
   hipLaunchKernelGGL(k_fill_u64, dim3((unsigned) nb), dim3(256), 0, st, p, n);
 }
-#endif  // HYDRO_MIXED_UNIT
+#endif  // HYDRO_BASE_UNIT
 
+// The arithmetic form of this unit's batched sweeps.
 template <typename S>
-int batch_sweep_occupancy(int kind, int waves) {
+constexpr int unit_form() {
+#ifdef HYDRO_FAST_UNIT
+  return kFast;
+#else
+  return sizeof(S) == 8 ? kGated : kIeee;
+#endif
+}
+
+template <typename S, int FD>
+static int occupancy_fd(int kind, int waves) {
   int nb = 0;
   const hipError_t e =
-      kind == 1
-          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_sweep_x<S, true>, 64 * waves, 0)
-          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_sweep_y<S, true>, 64 * waves,
-                                                         waves * sizeof(YStash<S>));
+      kind == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                      &nb, k_batch_sweep_x<S, true, FD>, 64 * waves, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                      &nb, k_batch_sweep_y<S, true, FD>, 64 * waves, waves * sizeof(YStash<S>));
   if (e != hipSuccess) return 2;
   return nb > 0 ? nb : 1;
 }
-#ifndef HYDRO_MIXED_UNIT
-template int batch_sweep_occupancy<double>(int, int);
-#else
-template int batch_sweep_occupancy<float>(int, int);
-#endif
 
-template <typename S, bool ST1>
+template <typename S, bool ST1, int FD>
 static void launch_sweep_part(int kind, const InstDev<S>* inst, const BTask* tasks, int ntask,
                               int phase, const BatchTune& tn, int nblocks, uint64_t* q,
                               hipStream_t st) {
   if (ntask <= 0) return;
   const int nb = ntask < nblocks ? ntask : nblocks;
   if (kind == 1)
-    hipLaunchKernelGGL((k_batch_sweep_x<S, ST1>), dim3(nb), dim3(64 * tn.waves), 0, st, inst,
+    hipLaunchKernelGGL((k_batch_sweep_x<S, ST1, FD>), dim3(nb), dim3(64 * tn.waves), 0, st, inst,
                        tasks, ntask, phase, tn.xiter, q);
   else
-    hipLaunchKernelGGL((k_batch_sweep_y<S, ST1>), dim3(nb), dim3(64 * tn.waves),
+    hipLaunchKernelGGL((k_batch_sweep_y<S, ST1, FD>), dim3(nb), dim3(64 * tn.waves),
                        tn.waves * sizeof(YStash<S>), st, inst, tasks, ntask, phase, tn.yseg, q);
+}
+
+template <typename S, int FD>
+static void launch_batch_sweep_fd(int kind, const InstDev<S>* inst, const BTask* tasks,
+                                  int ntask, int nst1, int phase, const BatchTune& tn,
+                                  int nblocks, uint64_t* q, hipStream_t st) {
+  launch_sweep_part<S, true, FD>(kind, inst, tasks, nst1, phase, tn, nblocks, q, st);
+  launch_sweep_part<S, false, FD>(kind, inst, tasks + nst1, ntask - nst1, phase, tn, nblocks,
+                                  q + 1, st);
+}
+
+#ifndef HYDRO_FAST_UNIT
+template <typename S>
+int batch_sweep_occupancy(int kind, int waves) {
+  return occupancy_fd<S, unit_form<S>()>(kind, waves);
 }
 
 template <typename S>
 void launch_batch_sweep(int kind, const InstDev<S>* inst, const BTask* tasks, int ntask,
                         int nst1, int phase, const BatchTune& tn, int nblocks, uint64_t* q,
                         hipStream_t st) {
-  launch_sweep_part<S, true>(kind, inst, tasks, nst1, phase, tn, nblocks, q, st);
-  launch_sweep_part<S, false>(kind, inst, tasks + nst1, ntask - nst1, phase, tn, nblocks, q + 1,
-                              st);
+  launch_batch_sweep_fd<S, unit_form<S>()>(kind, inst, tasks, ntask, nst1, phase, tn, nblocks, q,
+                                           st);
 }
+#ifdef HYDRO_BASE_UNIT
+template int batch_sweep_occupancy<double>(int, int);
+#else
+template int batch_sweep_occupancy<float>(int, int);
+#endif
+#else
+int batch_sweep_occupancy_fast(int kind, int waves) {
+  return occupancy_fd<double, kFast>(kind, waves);
+}
+
+void launch_batch_sweep_fast(int kind, const InstDev<double>* inst, const BTask* tasks,
+                             int ntask, int nst1, int phase, const BatchTune& tn, int nblocks,
+                             uint64_t* q, hipStream_t st) {
+  launch_batch_sweep_fd<double, kFast>(kind, inst, tasks, ntask, nst1, phase, tn, nblocks, q, st);
+}
+#endif
 
 template <typename S>
 void launch_batch_boundary(const InstDev<S>* inst, const BTask* tasks, int ntask, int phase,
@@ -313,11 +347,11 @@ void launch_batch_ctl(const InstDev<S>* inst, const int* ids, int nids, bool beg
                                          hipStream_t);                                           \
   template void launch_batch_ctl<S>(const InstDev<S>*, const int*, int, bool, uint64_t*,      \
                                     hipStream_t);
-#ifndef HYDRO_MIXED_UNIT
+#if defined(HYDRO_BASE_UNIT)
 INST_ARITH(double)
 INST_MOVE(double)
 INST_MOVE(float)
-#else
+#elif defined(HYDRO_MIXED_UNIT)
 INST_ARITH(float)
 #endif
 #undef INST_ARITH

@@ -59,5 +59,12 @@ void launch_batch_set_ctl(RunCtl* c, double t, double dt, double tend, int nstep
 void launch_zero_u64(uint64_t* p, long n, hipStream_t st);
 template <typename S>
 int batch_sweep_occupancy(int kind, int waves);  // resident workgroups per CU
+// The FP64 fast form's batched sweeps (hydro_batch.hip, fast unit): as
+// launch_batch_sweep / batch_sweep_occupancy for double, with the kFast
+// arithmetic.
+void launch_batch_sweep_fast(int kind, const InstDev<double>* inst, const BTask* tasks,
+                             int ntask, int nst1, int phase, const BatchTune& tn, int nblocks,
+                             uint64_t* q, hipStream_t st);
+int batch_sweep_occupancy_fast(int kind, int waves);
 
 }  // namespace hydro

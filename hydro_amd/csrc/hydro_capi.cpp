@@ -95,6 +95,9 @@ struct hydro_amd_grid {
   long prof_n[4] = {0, 0, 0, 0};
 };
 
+// FP64 storage (the FP64 and FP64 fast forms) rather than FP32.
+static bool dbl(const hydro_amd_grid* g) { return g->prec != HYDRO_AMD_MIXED; }
+
 static int ev_pair(hydro_amd_grid* g) {
   if (g->ev_used + 2 > (int) g->ev_pool.size()) {
     for (int k = 0; k < 64; k++) {
@@ -219,7 +222,7 @@ extern "C" void hydro_amd_grid_destroy(hydro_amd_grid* g) {
 static int alloc_buffer(hydro_amd_grid* g, int k) {
   const long total = 4 * g->plane + kOrigin + 64;
   HIPCHK(hipMalloc(&g->mem[k], (size_t) total * g->esz));
-  if (g->prec == HYDRO_AMD_FP64) {
+  if (dbl(g)) {
     double* b = (double*) g->mem[k] + kOrigin;
     for (int v = 0; v < 4; v++) g->vd[k].p[v] = b + v * g->plane;
   } else {
@@ -239,7 +242,8 @@ static int validate(const hydro_amd_params* p) {
     return fail("grid too large");
   if (!(p->dx > 0)) return fail("dx must be > 0");
   if (!(p->gamma > 1)) return fail("gamma must be > 1");
-  if (p->precision != HYDRO_AMD_FP64 && p->precision != HYDRO_AMD_MIXED)
+  if (p->precision != HYDRO_AMD_FP64 && p->precision != HYDRO_AMD_MIXED &&
+      p->precision != HYDRO_AMD_FP64_FAST)
     return fail("unknown precision %d", p->precision);
   if (p->scheme < 1 || p->scheme > 3) return fail("unknown scheme %d", p->scheme);
   if (p->niter_riemann < 0) return fail("niter_riemann must be >= 0");
@@ -251,10 +255,10 @@ extern "C" hydro_amd_grid* hydro_amd_grid_create(const hydro_amd_params* p) {
   hydro_amd_grid* g = new hydro_amd_grid();
   g->p = *p;
   g->prec = p->precision;
-  g->esz = g->prec == HYDRO_AMD_FP64 ? sizeof(double) : sizeof(float);
+  g->esz = dbl(g) ? sizeof(double) : sizeof(float);
   // the 4-wave LDS x form in both precisions (FP64: within 2 % of the DPP
   // form at 8192^2 and faster on small grids; mixed: its fastest form)
-  g->tune = g->prec == HYDRO_AMD_FP64 ? SweepTune{64, 1, 4, 8} : SweepTune{64, 4, 4, 16};
+  g->tune = dbl(g) ? SweepTune{64, 1, 4, 8} : SweepTune{64, 4, 4, 16};
   g->geom.nx = p->nx;
   g->geom.ny = p->ny;
   g->geom.pitch = ((long) p->nx + 4 + 31) / 32 * 32;
@@ -313,7 +317,7 @@ static int write_ctl(hydro_amd_grid* g) {
 static void bump(hydro_amd_grid* g) { g->state_ver++; }
 
 static int do_init(hydro_amd_grid* g, int bx, int by0, int by1, double e) {
-  if (g->prec == HYDRO_AMD_FP64)
+  if (dbl(g))
     launch_init<double>(g->geom, g->vd[0], g->vd[1], bx, by0, by1, e, g->st);
   else
     launch_init<float>(g->geom, g->vf[0], g->vf[1], bx, by0, by1, e, g->st);
@@ -348,7 +352,7 @@ extern "C" int hydro_amd_grid_upload(hydro_amd_grid* g, const double* u) {
   if (!g || !u) return fail("null argument");
   const long nxt = g->p.nx + 4, rows = 4L * (g->p.ny + 4);
   void* dst = (char*) g->mem[0] + kOrigin * g->esz;
-  if (g->prec == HYDRO_AMD_FP64) {
+  if (dbl(g)) {
     HIPCHK(hipMemcpy2DAsync(dst, g->geom.pitch * 8, u, nxt * 8, nxt * 8, rows,
                             hipMemcpyHostToDevice, g->st));
   } else {
@@ -393,7 +397,7 @@ extern "C" int hydro_amd_grid_image(hydro_amd_grid* g, int shrink_size, unsigned
   HIPCHK(hipMallocAsync(&d, (size_t) (8 + need), g->st));
   unsigned long long* key = (unsigned long long*) d;
   unsigned char* px = (unsigned char*) d + 8;
-  if (g->prec == HYDRO_AMD_FP64)
+  if (dbl(g))
     launch_image<double>(g->geom, g->vd[0].p[0], s, w, h, key, px, g->st);
   else
     launch_image<float>(g->geom, g->vf[0].p[0], s, w, h, key, px, g->st);
@@ -409,7 +413,7 @@ extern "C" int hydro_amd_grid_image(hydro_amd_grid* g, int shrink_size, unsigned
 
 static void enq_boundary(hydro_amd_grid* g, int idim, int src, const RunCtl* ctl) {
   timed(g, 2, [&] {
-    if (g->prec == HYDRO_AMD_FP64)
+    if (dbl(g))
       launch_boundary<double>(idim, g->geom, g->ph, g->vd[src], g->vd[1 - src], ctl, g->st);
     else
       launch_boundary<float>(idim, g->geom, g->ph, g->vf[src], g->vf[1 - src], ctl, g->st);
@@ -420,7 +424,10 @@ static void enq_sweep(hydro_amd_grid* g, int idim, int src, double dt, const Run
                       bool cour) {
   uint64_t* c = cour ? g->cour : nullptr;
   timed(g, idim == 1 ? 0 : 1, [&] {
-    if (g->prec == HYDRO_AMD_FP64)
+    if (g->prec == HYDRO_AMD_FP64_FAST)
+      launch_sweep_fast(idim, g->geom, g->ph, g->vd[src], g->vd[1 - src], dt, ctl, c, g->tune,
+                        g->st);
+    else if (g->prec == HYDRO_AMD_FP64)
       launch_sweep<double>(idim, g->geom, g->ph, g->vd[src], g->vd[1 - src], dt, ctl, c,
                            g->tune, g->st);
     else
@@ -431,7 +438,7 @@ static void enq_sweep(hydro_amd_grid* g, int idim, int src, double dt, const Run
 
 static void enq_courant(hydro_amd_grid* g, const RunCtl* ctl) {
   timed(g, 3, [&] {
-    if (g->prec == HYDRO_AMD_FP64)
+    if (dbl(g))  // (the FP64 fast form too: one launch per state, IEEE arithmetic)
       launch_courant<double>(g->geom, g->ph, g->vd[0], g->cour, ctl, g->st);
     else
       launch_courant<float>(g->geom, g->ph, g->vf[0], g->cour, ctl, g->st);
@@ -715,7 +722,10 @@ static double time_sweep(hydro_amd_grid* g, int idim, const SweepTune& t, int re
     return 1e30;
   }
   auto launch = [&] {
-    if (g->prec == HYDRO_AMD_FP64)
+    if (g->prec == HYDRO_AMD_FP64_FAST)
+      launch_sweep_fast(idim, g->geom, g->ph, g->vd[0], g->vd[1], 1e-12, nullptr, cour, t,
+                        g->st);
+    else if (g->prec == HYDRO_AMD_FP64)
       launch_sweep<double>(idim, g->geom, g->ph, g->vd[0], g->vd[1], 1e-12, nullptr, cour, t,
                            g->st);
     else

@@ -3,14 +3,16 @@
 // state and the device-side time-loop control.  The per-task bodies live in
 // hydro_sweeps.hpp (shared with the batched kernels).
 //
-// The file is compiled twice.  The FP64 unit (the default) holds everything
-// except the FP32-storage sweeps and CFL kernels and is built with
-// -ffp-contract=off, so the FP64 path is op-for-op the reference's.  The
-// mixed-precision unit (HYDRO_MIXED_UNIT defined) holds only those FP32
-// kernels and is built with -ffp-contract=on: the mixed form carries no
-// bitwise contract with the reference, and multiply-adds fused within each
-// source expression cut its VALU instruction count while keeping the
-// single-grid and batched kernels (same source) bit-identical.
+// The file is compiled three times.  The FP64 unit (the default,
+// HYDRO_BASE_UNIT) holds everything except the FP32-storage and FP64 fast
+// sweeps and is built with -ffp-contract=off, so the FP64 path is op-for-op
+// the reference's.  The mixed-precision unit (HYDRO_MIXED_UNIT) holds only
+// the FP32-storage sweeps and CFL kernels, and the fast unit
+// (HYDRO_FAST_UNIT) only the FP64 fast-form sweeps; both are built with
+// -ffp-contract=on: those forms carry no bitwise contract with the
+// reference, and multiply-adds fused within each source expression cut their
+// VALU instruction counts while keeping the single-grid and batched kernels
+// (same source) bit-identical.
 #include "hydro_kernels.hpp"
 #include "hydro_sweeps.hpp"
 
@@ -19,7 +21,7 @@ namespace hydro {
 // ---- x sweep ---------------------------------------------------------------
 
 // A workgroup walks wpb * xiter consecutive row segments, wpb at a time.
-template <typename S, bool ST1, bool COUR, bool FD>
+template <typename S, bool ST1, bool COUR, int FD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_sweep_x(GridGeom g, Phys<S> ph, double dx,
                                                  double dt_val, const RunCtl* ctl,
                                                  GridView<S> in, GridView<S> out,
@@ -44,7 +46,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 
 // Workgroup form (W = 2 or 4 waves): a workgroup walks xiter consecutive
 // row pieces of 64*W - 4 updated cells, neighbours through LDS.
-template <typename S, bool ST1, bool COUR, int W, bool FD>
+template <typename S, bool ST1, bool COUR, int W, int FD>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) void k_sweep_xb(GridGeom g, Phys<S> ph, double dx,
                                                      double dt_val, const RunCtl* ctl,
                                                      GridView<S> in, GridView<S> out,
@@ -70,7 +72,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
 
 // A workgroup covers 64 columns; its waves take consecutive row segments of
 // yseg rows each.
-template <typename S, bool ST1, bool COUR, bool FD>
+template <typename S, bool ST1, bool COUR, int FD>
 __global__ __launch_bounds__(256) void k_sweep_y(GridGeom g, Phys<S> ph, double dx,
                                                  double dt_val, const RunCtl* ctl,
                                                  GridView<S> in, GridView<S> out,
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(256) void k_boundary(int idim, GridGeom g, int bl, 
   boundary_line<S>(idim, g, bl, br, bd, bu, src.p[v], has_dst2 ? dst2.p[v] : nullptr, v, k);
 }
 
-#ifndef HYDRO_MIXED_UNIT
+#ifdef HYDRO_BASE_UNIT
 // ---- dt and run control ------------------------------------------------------
 
 __global__ __launch_bounds__(64) void k_dt_finalize(uint64_t* cour, double cf, double dx,
@@ -175,7 +177,7 @@ __global__ void k_ctl_end(RunCtl* ctl) {
 __global__ void k_clear_slots(uint64_t* cour) {
   for (int s = threadIdx.x; s < 2 * kCourSlots; s += blockDim.x) cour[s] = 0;
 }
-#endif  // HYDRO_MIXED_UNIT
+#endif  // HYDRO_BASE_UNIT
 
 template <typename S>
 __global__ __launch_bounds__(256) void k_init(GridGeom g, GridView<S> a, GridView<S> b, int bx,
@@ -185,7 +187,7 @@ __global__ __launch_bounds__(256) void k_init(GridGeom g, GridView<S> a, GridVie
   init_cell<S>(g, a, b, t, bx, by0, by1, eb);
 }
 
-#ifndef HYDRO_MIXED_UNIT
+#ifdef HYDRO_BASE_UNIT
 // ---- plain copy (measurement reference) ---------------------------------------
 
 // Streams n doubles from a to b; used to calibrate HBM byte counters and as
@@ -200,7 +202,7 @@ __global__ __launch_bounds__(256) void k_copy_probe(const double* __restrict__ a
 void launch_copy_probe(const double* a, double* b, long n, hipStream_t st) {
   hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, st, a, b, n);
 }
-#endif  // HYDRO_MIXED_UNIT
+#endif  // HYDRO_BASE_UNIT
 
 // ---- launchers -----------------------------------------------------------------
 
@@ -216,33 +218,19 @@ void launch_boundary(int idim, const GridGeom& g, const PhysArgs& ph, GridView<S
                      ph.bd, ph.bu, src, dst2, has2, ctl);
 }
 
-template <typename S>
-void launch_sweep(int idim, const GridGeom& g, const PhysArgs& pa, GridView<S> in,
-                  GridView<S> out, double dt_val, const RunCtl* ctl, uint64_t* cour,
-                  const SweepTune& tune, hipStream_t st) {
-  const Phys<S> ph = make_phys<S>(pa.gamma, pa.smallr, pa.smallc, pa.slope_type, pa.niter,
-                                  pa.iorder, pa.scheme);
-  const bool st1 = pa.slope_type == 1.0;
+// The sweep launch for one arithmetic form FD (shapes as in launch_sweep).
+template <typename S, int FD>
+static void launch_sweep_fd(int idim, const GridGeom& g, const Phys<S>& ph, double dx,
+                            GridView<S> in, GridView<S> out, double dt_val, const RunCtl* ctl,
+                            uint64_t* cour, const SweepTune& tune, bool st1, hipStream_t st) {
   const bool c = cour != nullptr;
-  // The range-gated FP64 divisions (the kernels' FD variants) pay when the
-  // launch keeps the SIMDs busy.  A sweep of a small grid has too few waves
-  // for that: each wave's march is latency-bound, and the gates' wave-wide
-  // tests, branches and larger code lengthen it, so such launches run the
-  // variants without them (the results are the same bits either way).
-  constexpr long kFastDivWaves = 4096;  // 4 waves per SIMD on 256 CUs
-  constexpr bool kFD = sizeof(S) == 8;  // FP32 storage has no FD variants
-  const bool fd = kFD && ph.fastdiv &&
-                  (idim == 1 ? (long) cdiv(g.nx, kXSeg) * g.ny
-                             : (long) cdiv(g.nx, 64) * cdiv(g.ny, 16)) >= kFastDivWaves;
-#define DISPATCH(F)                                        \
-  do {                                                     \
-    if (st1) {                                             \
-      if (c) { if (fd) F(true, true, kFD); else F(true, true, false); }    \
-      else { if (fd) F(true, false, kFD); else F(true, false, false); }   \
-    } else {                                               \
-      if (c) { if (fd) F(false, true, kFD); else F(false, true, false); }  \
-      else { if (fd) F(false, false, kFD); else F(false, false, false); } \
-    }                                                      \
+#define DISPATCH(F)                         \
+  do {                                      \
+    if (st1) {                              \
+      if (c) F(true, true); else F(true, false);    \
+    } else {                                \
+      if (c) F(false, true); else F(false, false);  \
+    }                                       \
   } while (0)
   if (idim == 1 && (tune.xwaves == 2 || tune.xwaves == 4)) {
     const int W = tune.xwaves;
@@ -251,10 +239,10 @@ void launch_sweep(int idim, const GridGeom& g, const PhysArgs& pa, GridView<S> i
     int xiter = tune.xiter > 0 ? tune.xiter : 8;
     while (xiter > 1 && cdiv(pieces, xiter) < 1024) xiter >>= 1;
     const dim3 grid(cdiv(pieces, xiter)), blk(64 * W);
-#define HXB(A, B, WW, F) \
-  hipLaunchKernelGGL((k_sweep_xb<S, A, B, WW, F>), grid, blk, 0, st, g, ph, pa.dx, dt_val, ctl, \
+#define HXB(A, B, WW) \
+  hipLaunchKernelGGL((k_sweep_xb<S, A, B, WW, FD>), grid, blk, 0, st, g, ph, dx, dt_val, ctl, \
                      in, out, cour, nseg, xiter)
-#define HXW(A, B, F) do { if (W == 2) HXB(A, B, 2, F); else HXB(A, B, 4, F); } while (0)
+#define HXW(A, B) do { if (W == 2) HXB(A, B, 2); else HXB(A, B, 4); } while (0)
     DISPATCH(HXW);
 #undef HXW
 #undef HXB
@@ -266,8 +254,8 @@ void launch_sweep(int idim, const GridGeom& g, const PhysArgs& pa, GridView<S> i
     // keep at least ~4 workgroups per CU before walking several segments
     while (xiter > 1 && cdiv(waves, (long) xw * xiter) < 1024) xiter >>= 1;
     const dim3 grid(cdiv(waves, (long) xw * xiter)), blk(64 * xw);
-#define HX(A, B, F) \
-  hipLaunchKernelGGL((k_sweep_x<S, A, B, F>), grid, blk, 0, st, g, ph, pa.dx, dt_val, ctl, in, \
+#define HX(A, B) \
+  hipLaunchKernelGGL((k_sweep_x<S, A, B, FD>), grid, blk, 0, st, g, ph, dx, dt_val, ctl, in, \
                      out, cour, nseg, xiter)
     DISPATCH(HX);
 #undef HX
@@ -279,18 +267,53 @@ void launch_sweep(int idim, const GridGeom& g, const PhysArgs& pa, GridView<S> i
     // (down to 16 rows; each adds a 4-row halo) until there are about two
     // waves per SIMD, rather than leave most of the GPU idle
     while (yseg > 16 && (long) ncolblk * cdiv(g.ny, yseg) < 2048) yseg >>= 1;
-#define HY(A, B, F)                                                                            \
-  do {                                                                                         \
-    const int nsg = (int) cdiv(g.ny, (long) yseg * yw);                                        \
-    hipLaunchKernelGGL((k_sweep_y<S, A, B, F>), dim3((unsigned) ncolblk * nsg), dim3(64 * yw),  \
-                       yw * sizeof(YStash<S>), st, g, ph, pa.dx, dt_val, ctl, in, out, cour,   \
-                       yseg, ncolblk);                                                          \
-  } while (0)
+    const int nsg = (int) cdiv(g.ny, (long) yseg * yw);
+#define HY(A, B)                                                                             \
+  hipLaunchKernelGGL((k_sweep_y<S, A, B, FD>), dim3((unsigned) ncolblk * nsg), dim3(64 * yw), \
+                     yw * sizeof(YStash<S>), st, g, ph, dx, dt_val, ctl, in, out, cour, yseg,  \
+                     ncolblk)
     DISPATCH(HY);
 #undef HY
   }
 #undef DISPATCH
 }
+
+#ifndef HYDRO_FAST_UNIT
+template <typename S>
+void launch_sweep(int idim, const GridGeom& g, const PhysArgs& pa, GridView<S> in,
+                  GridView<S> out, double dt_val, const RunCtl* ctl, uint64_t* cour,
+                  const SweepTune& tune, hipStream_t st) {
+  const Phys<S> ph = make_phys<S>(pa.gamma, pa.smallr, pa.smallc, pa.slope_type, pa.niter,
+                                  pa.iorder, pa.scheme);
+  const bool st1 = pa.slope_type == 1.0;
+  // The range-gated FP64 divisions (the kGated kernels) pay when the launch
+  // keeps the SIMDs busy.  A sweep of a small grid has too few waves for
+  // that: each wave's march is latency-bound, and the gates' wave-wide
+  // tests, branches and larger code lengthen it, so such launches run the
+  // kIeee kernels (the results are the same bits either way).
+  constexpr long kFastDivWaves = 4096;  // 4 waves per SIMD on 256 CUs
+  if constexpr (sizeof(S) == 8) {
+    const bool fd = ph.fastdiv &&
+                    (idim == 1 ? (long) cdiv(g.nx, kXSeg) * g.ny
+                               : (long) cdiv(g.nx, 64) * cdiv(g.ny, 16)) >= kFastDivWaves;
+    if (fd)
+      launch_sweep_fd<S, kGated>(idim, g, ph, pa.dx, in, out, dt_val, ctl, cour, tune, st1, st);
+    else
+      launch_sweep_fd<S, kIeee>(idim, g, ph, pa.dx, in, out, dt_val, ctl, cour, tune, st1, st);
+  } else {
+    launch_sweep_fd<S, kIeee>(idim, g, ph, pa.dx, in, out, dt_val, ctl, cour, tune, st1, st);
+  }
+}
+#else
+void launch_sweep_fast(int idim, const GridGeom& g, const PhysArgs& pa, GridView<double> in,
+                       GridView<double> out, double dt_val, const RunCtl* ctl, uint64_t* cour,
+                       const SweepTune& tune, hipStream_t st) {
+  const Phys<double> ph = make_phys<double>(pa.gamma, pa.smallr, pa.smallc, pa.slope_type,
+                                            pa.niter, pa.iorder, pa.scheme);
+  launch_sweep_fd<double, kFast>(idim, g, ph, pa.dx, in, out, dt_val, ctl, cour, tune,
+                                 pa.slope_type == 1.0, st);
+}
+#endif
 
 template <typename S>
 void launch_courant(const GridGeom& g, const PhysArgs& pa, GridView<S> in, uint64_t* cour,
@@ -304,7 +327,7 @@ void launch_courant(const GridGeom& g, const PhysArgs& pa, GridView<S> in, uint6
   hipLaunchKernelGGL(k_courant<S>, dim3((unsigned) nb), dim3(256), 0, st, g, ph, in, cour, ctl);
 }
 
-#ifndef HYDRO_MIXED_UNIT
+#ifdef HYDRO_BASE_UNIT
 void launch_dt_finalize(uint64_t* cour, double cf, double dx, double smallc, double* dt_out,
                         hipStream_t st) {
   hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, cour, cf, dx, smallc, dt_out);
@@ -322,7 +345,7 @@ void launch_ctl_end(RunCtl* ctl, hipStream_t st) {
 void launch_clear_slots(uint64_t* cour, hipStream_t st) {
   hipLaunchKernelGGL(k_clear_slots, dim3(1), dim3(256), 0, st, cour);
 }
-#endif  // HYDRO_MIXED_UNIT
+#endif  // HYDRO_BASE_UNIT
 
 template <typename S>
 void launch_init(const GridGeom& g, GridView<S> a, GridView<S> b, int bx, int by0, int by1,
@@ -331,7 +354,7 @@ void launch_init(const GridGeom& g, GridView<S> a, GridView<S> b, int bx, int by
   hipLaunchKernelGGL(k_init<S>, dim3(cdiv(n, 256)), dim3(256), 0, st, g, a, b, bx, by0, by1,
                      (S) e_blast);
 }
-#ifndef HYDRO_MIXED_UNIT
+#ifdef HYDRO_BASE_UNIT
 template void launch_init<double>(const GridGeom&, GridView<double>, GridView<double>, int, int,
                                   int, double, hipStream_t);
 template void launch_init<float>(const GridGeom&, GridView<float>, GridView<float>, int, int,
@@ -345,7 +368,7 @@ template void launch_sweep<double>(int, const GridGeom&, const PhysArgs&, GridVi
                                    const SweepTune&, hipStream_t);
 template void launch_courant<double>(const GridGeom&, const PhysArgs&, GridView<double>,
                                      uint64_t*, const RunCtl*, hipStream_t);
-#else
+#elif defined(HYDRO_MIXED_UNIT)
 template void launch_sweep<float>(int, const GridGeom&, const PhysArgs&, GridView<float>,
                                   GridView<float>, double, const RunCtl*, uint64_t*,
                                   const SweepTune&, hipStream_t);

@@ -3,6 +3,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// The kernel files (hydro_kernels.hip, hydro_batch.hip) are compiled once per
+// arithmetic unit: HYDRO_MIXED_UNIT (FP32-storage sweeps), HYDRO_FAST_UNIT
+// (FP64 fast-form sweeps) or neither (everything else, FP64 bitwise).
+#if !defined(HYDRO_MIXED_UNIT) && !defined(HYDRO_FAST_UNIT)
+#define HYDRO_BASE_UNIT 1
+#endif
+
 namespace hydro {
 
 // Device-resident time-loop state (HydroC99_2DMpi/Src/main.c:244-298 moved
@@ -61,6 +68,12 @@ template <typename S>
 void launch_sweep(int idim, const GridGeom& g, const PhysArgs& ph, GridView<S> in,
                   GridView<S> out, double dt_val, const RunCtl* ctl, uint64_t* cour,
                   const SweepTune& tune, hipStream_t st);
+
+// The FP64 fast form's sweeps (hydro_kernels.hip, fast unit): as
+// launch_sweep, with the kFast arithmetic.
+void launch_sweep_fast(int idim, const GridGeom& g, const PhysArgs& ph, GridView<double> in,
+                       GridView<double> out, double dt_val, const RunCtl* ctl, uint64_t* cour,
+                       const SweepTune& tune, hipStream_t st);
 
 template <typename S>
 void launch_courant(const GridGeom& g, const PhysArgs& ph, GridView<S> in, uint64_t* cour,

@@ -12,12 +12,28 @@
 // speeds and fluxes are float; the Riemann solver's Newton pressure
 // iteration runs in double (riemann_flux), the update accumulates in double
 // (update1) and the CFL maximum and dt are double in the callers.
+//
+// FP64 fast form (R = double, form kFast): FP64 throughout, but not bit for
+// bit: quotients and square roots come from the hardware estimates refined to
+// within about an ulp (shared reciprocals, no correctly rounded sequences),
+// the Newton iteration is rearranged to one division per step, and the unit
+// that holds these kernels is compiled with multiply-add contraction.  Its
+// results stay within a few ulp per operation of the reference's.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
 
 namespace hydro {
+
+// Arithmetic forms of the FP64 sweeps (the FD template argument of the sweep
+// bodies): kIeee -- the IEEE operations alone; kGated -- the range-gated
+// divisions (FastDiv below), bit-identical to kIeee; kFast -- the FP64 fast
+// form (estimate-based quotients and roots, see prim_fast / riemann_fast).
+// The FP32-storage sweeps take kIeee (their arithmetic is chosen by R).
+constexpr int kIeee = 0;
+constexpr int kGated = 1;
+constexpr int kFast = 2;
 
 // Physical / scheme constants, precomputed per launch in the compute type.
 template <typename R>
@@ -66,7 +82,8 @@ template <typename R>
 struct Prim {
   Q4<R> q;
   R c;     // sound speed
-  R rinv;  // FP64 with FAST: the refined reciprocal of q.r (FastDiv)
+  R rinv;  // FP64 with FAST / kFast: the refined reciprocal of q.r
+  R cinv;  // kFast: 1 / c
 };
 
 template <typename R>
@@ -279,6 +296,7 @@ template <typename R, bool FAST = false, bool UABS = false>
 __device__ __forceinline__ Prim<R> prim_cell(R U0, R U1, R U2, R U3, const Phys<R>& ph) {
   Prim<R> o;
   o.rinv = (R) 0;
+  o.cinv = (R) 0;
   const R half = (R) 0.5;
   R r = maxm(U0, ph.smallr);
   R u, v, e, p;
@@ -327,15 +345,80 @@ __device__ __forceinline__ bool wave_cell_in(S U0, S U1, S U2, S U3, const Phys<
   }
 }
 
+// ---- FP64 fast form (kFast) ---------------------------------------------------
+//
+// Quotients and square roots from the hardware estimates v_rcp_f64 and
+// v_rsq_f64 (relative error about 2^-24, measured on MI355X) refined by
+// Newton-Raphson or coupled Goldschmidt steps instead of the correctly rounded sequences: two steps
+// (rcp2, sqrt_rsq2, sqrt_1ulp) leave a result within about an ulp, and
+// divisions by one denominator share its reciprocal.  Inside the Newton
+// pressure iteration one step (rcp_nr, sqrt_rsq_nr: about 2^-48) is enough:
+// an error in an iterate is damped by the next Newton step, so p* carries
+// only the error of its last correction, which is below 1e-6 of p*.
+
+// 1/b within about an ulp: the estimate and two Newton-Raphson steps.
+__device__ __forceinline__ double rcp2(double b) {
+  const double y0 = __builtin_amdgcn_rcp(b);
+  const double y1 = __builtin_fma(__builtin_fma(-b, y0, 1.0), y0, y0);
+  return __builtin_fma(__builtin_fma(-b, y1, 1.0), y1, y1);
+}
+// sqrt(x) and 1/sqrt(x) for x > 0, both within about an ulp: two coupled
+// Goldschmidt steps from one v_rsq_f64 estimate g (s = x g, h = g / 2,
+// r = 1/2 - s h, then s += s r, h += h r).
+__device__ __forceinline__ void sqrt_rsq2(double x, double& s, double& is) {
+  const double g = __builtin_amdgcn_rsq(x);
+  const double s0 = x * g, h0 = 0.5 * g;
+  const double r0 = __builtin_fma(-h0, s0, 0.5);
+  const double s1 = __builtin_fma(s0, r0, s0), h1 = __builtin_fma(h0, r0, h0);
+  const double r1 = __builtin_fma(-h1, s1, 0.5);
+  s = __builtin_fma(s1, r1, s1);
+  is = 2.0 * __builtin_fma(h1, r1, h1);
+}
+// sqrt(x) for x >= 0 within about an ulp: one coupled step, then a residual
+// correction (sqrt(+0) gives NaN here; callers clamp the result with fmax,
+// which returns the other operand for a NaN).
+__device__ __forceinline__ double sqrt_1ulp(double x) {
+  const double g = __builtin_amdgcn_rsq(x);
+  const double s0 = x * g, h0 = 0.5 * g;
+  const double r0 = __builtin_fma(-h0, s0, 0.5);
+  const double s1 = __builtin_fma(s0, r0, s0), h1 = __builtin_fma(h0, r0, h0);
+  return __builtin_fma(__builtin_fma(-s1, s1, x), h1, s1);
+}
+
+// constoprim.c:62-78 + equation_of_state.c:64-81 for one cell, fast form:
+// one reciprocal of rho for the four quotients, and c and 1/c (for the
+// trace) from one estimate.
+__device__ __forceinline__ Prim<double> prim_fast(double U0, double U1, double U2, double U3,
+                                                  const Phys<double>& ph) {
+  Prim<double> o;
+  const double r = maxm(U0, ph.smallr);
+  const double ir = rcp2(r);
+  const double u = U1 * ir, v = U2 * ir;
+  const double eken = 0.5 * (u * u + v * v);
+  const double e = U3 * ir - eken;
+  double p = ph.gm1 * r * e;
+  p = maxm(p, r * ph.smallp);
+  sqrt_rsq2(ph.gamma * p * ir, o.c, o.cinv);  // gamma p / rho > 0
+  o.q = Q4<double>{r, u, v, p};
+  o.rinv = ir;
+  return o;
+}
+
 // Courant numbers of one cell in grid slot order (compute_deltat.c:71-88 +
-// courantOnXY.c:115-121): returns c + |vx| and c + |vy|.  FD: the primitives
-// take FastDiv when every active lane's cell passes cell_in.
-template <typename R, bool FD = false>
+// courantOnXY.c:115-121): returns c + |vx| and c + |vy|.  FD (the form):
+// kGated -- the primitives take FastDiv when every active lane's cell passes
+// cell_in; kFast -- prim_fast.
+template <typename R, int FD = kIeee>
 __device__ __forceinline__ void courant_cell(R U0, R U1, R U2, R U3, const Phys<R>& ph, R& cx,
                                              R& cy) {
-  const Prim<R> o = FD && wave_cell_in<R>(U0, U1, U2, U3, ph)
-                        ? prim_cell<R, true, true>(U0, U1, U2, U3, ph)
-                        : prim_cell<R, false>(U0, U1, U2, U3, ph);
+  Prim<R> o;
+  if constexpr (FD == kFast && sizeof(R) == 8) {
+    o = prim_fast(U0, U1, U2, U3, ph);
+  } else {
+    o = FD == kGated && wave_cell_in<R>(U0, U1, U2, U3, ph)
+            ? prim_cell<R, true, true>(U0, U1, U2, U3, ph)
+            : prim_cell<R, false>(U0, U1, U2, U3, ph);
+  }
   cx = o.c + fabs(o.q.u);
   cy = o.c + fabs(o.q.v);
 }
@@ -395,43 +478,24 @@ __device__ __forceinline__ TraceK<R> make_tracek(R dtdx, int scheme) {
   return k;
 }
 
-// trace.c:85-147: MUSCL-Hancock predictor of one cell -> qxm (the state at
-// the cell's right face, seen from inside) and qxp (its left face).
-// FAST (FP64): FastDiv, for cells whose own and neighbour values (the slope
-// inputs) passed cell_in.  RINV: rinv is prim_cell's reciprocal of q.r, reused
-// for cc / q.r.
-template <typename R, bool FAST = false, bool RINV = false>
-__device__ __forceinline__ void trace_cell(const Q4<R>& q, const Q4<R>& dq, R cc,
-                                           const TraceK<R>& k, Q4<R>& qxm, Q4<R>& qxp,
-                                           R rinv = (R) 0) {
+// trace.c:85-147 after its four quotients: the MUSCL-Hancock predictor of
+// one cell -> qxm (the state at the cell's right face, seen from inside) and
+// qxp (its left face).  rOcc = rho/c, OrOcc = c/rho, dprcc = dp/(rho c),
+// a0r = drho - dp/c^2, csq = c^2.
+template <typename R>
+__device__ __forceinline__ void trace_tail(const Q4<R>& q, const Q4<R>& dq, R cc, R csq, R rOcc,
+                                           R OrOcc, R dprcc, R a0r, const TraceK<R>& k,
+                                           Q4<R>& qxm, Q4<R>& qxp) {
   const R half = (R) 0.5;
   const R one = (R) 1;
-  R csq = cc * cc;
-  R rOcc, OrOcc, dprcc, a0r;
-  if constexpr (sizeof(R) == 8) {
-    using P = std::conditional_t<FAST, FastDiv, IeeeDiv>;
-    rOcc = P::divn(q.r, cc);  // rho, c > 0
-    if constexpr (FAST && RINV) OrOcc = qdiv_nz(cc, Rcp{q.r, rinv});
-    else OrOcc = P::divn(cc, q.r);
-    dprcc = P::div(dq.p, q.r * cc);
-    a0r = dq.r - P::div(dq.p, csq);
-  } else {
-    // two reciprocals for the four quotients
-    const R rcc = __builtin_amdgcn_rcpf(cc);
-    const R rr = __builtin_amdgcn_rcpf(q.r);
-    rOcc = q.r * rcc;
-    OrOcc = cc * rr;
-    dprcc = (dq.p * rr) * rcc;
-    a0r = dq.r - (dq.p * rcc) * rcc;
-  }
   R am = half * (dprcc - dq.u) * rOcc;
   R ap = half * (dprcc + dq.u) * rOcc;
   R a0v = dq.v;
   R upcc = q.u + cc;
   R umcc = q.u - cc;
   {
-    // (umcc * dtdx + one: one multiply-add in FP32, the reference's
-    // product then sum in FP64)
+    // (umcc * dtdx + one: one multiply-add where the unit contracts, the
+    // reference's product then sum in the FP64 unit)
     R spm = (umcc >= k.zeror) ? k.project : umcc * k.dtdx + one;
     R spp = (upcc >= k.zeror) ? k.project : upcc * k.dtdx + one;
     R sp0 = (q.u >= k.zeror) ? k.project : q.u * k.dtdx + one;
@@ -457,6 +521,35 @@ __device__ __forceinline__ void trace_cell(const Q4<R>& q, const Q4<R>& dq, R cc
     qxm.v = q.v + (azv);
     qxm.p = q.p + (apl + aml) * csq;
   }
+}
+
+// trace.c:85-147 for one cell (quotients as the form computes them, then
+// trace_tail).  FAST (FP64): FastDiv, for cells whose own and neighbour
+// values (the slope inputs) passed cell_in.  RINV: rinv is prim_cell's
+// reciprocal of q.r, reused for cc / q.r.
+template <typename R, bool FAST = false, bool RINV = false>
+__device__ __forceinline__ void trace_cell(const Q4<R>& q, const Q4<R>& dq, R cc,
+                                           const TraceK<R>& k, Q4<R>& qxm, Q4<R>& qxp,
+                                           R rinv = (R) 0) {
+  R csq = cc * cc;
+  R rOcc, OrOcc, dprcc, a0r;
+  if constexpr (sizeof(R) == 8) {
+    using P = std::conditional_t<FAST, FastDiv, IeeeDiv>;
+    rOcc = P::divn(q.r, cc);  // rho, c > 0
+    if constexpr (FAST && RINV) OrOcc = qdiv_nz(cc, Rcp{q.r, rinv});
+    else OrOcc = P::divn(cc, q.r);
+    dprcc = P::div(dq.p, q.r * cc);
+    a0r = dq.r - P::div(dq.p, csq);
+  } else {
+    // two reciprocals for the four quotients
+    const R rcc = __builtin_amdgcn_rcpf(cc);
+    const R rr = __builtin_amdgcn_rcpf(q.r);
+    rOcc = q.r * rcc;
+    OrOcc = cc * rr;
+    dprcc = (dq.p * rr) * rcc;
+    a0r = dq.r - (dq.p * rcc) * rcc;
+  }
+  trace_tail<R>(q, dq, cc, csq, rOcc, OrOcc, dprcc, a0r, k, qxm, qxp);
 }
 
 // ---- FP64 Riemann solver (the FP64 form) ------------------------------------
@@ -604,6 +697,121 @@ __device__ __forceinline__ Q4<double> riemann_d(const Q4<double>& ql, const Q4<d
                             IeeeDiv::R{pr}, ph);
 }
 
+// trace.c:85-147, fast form: the four quotients from 1/rho and 1/c.
+__device__ __forceinline__ void trace_fast(const Q4<double>& q, const Q4<double>& dq, double cc,
+                                           double ic, double ir, const TraceK<double>& k,
+                                           Q4<double>& qxm, Q4<double>& qxp) {
+  const double dpic = dq.p * ic;
+  trace_tail<double>(q, dq, cc, cc * cc, q.r * ic, cc * ir, dpic * ir, dq.r - dpic * ic, k, qxm,
+                     qxp);
+}
+
+// riemann.c:128-276 + cmpflx.c:67-90 for one face, fast form.  The first
+// guess and the Newton iteration are the reference's recurrence with fewer
+// divisions (as in the mixed form, here in FP64 throughout): 1/pl and 1/pr
+// are loop invariants, 1/w comes from the estimate that gives w, with
+// q = a / b (a = 2 w^3, b = w^2 + c) qr ql / (qr + ql) = al ar / (al br +
+// ar bl) is one division, and |d / (p* + smallpp)| > 1e-6 is tested as
+// |d| > 1e-6 (p* + smallpp).  In the star state, sqrt(gamma po / ro) is
+// sqrt(gamma po ro) / ro with sqrt(gamma po ro) from the first guess, and
+// wo / ro, ro / (1 + ...) and (spout + spin) / scr are products with
+// reciprocals.
+__device__ __forceinline__ Q4<double> riemann_fast(const Q4<double>& ql, const Q4<double>& qr,
+                                                   const Phys<double>& ph) {
+  const double half = 0.5, one = 1.0;
+  const double rl = fmax(ql.r, ph.smallr);
+  const double ul = ql.u;
+  const double pl = fmax(ql.p, rl * ph.smallp);
+  const double rr = fmax(qr.r, ph.smallr);
+  const double ur = qr.u;
+  const double pr = fmax(qr.p, rr * ph.smallp);
+  const double cl = ph.gamma * pl * rl;
+  const double cr = ph.gamma * pr * rr;
+
+  // first guess (riemann.c:145-150); wl0 = sqrt(gamma pl rl) serves the star
+  // state's sound speed too, so it is refined to an ulp
+  double wl0, iwl0, wr0, iwr0;
+  sqrt_rsq2(cl, wl0, iwl0);
+  sqrt_rsq2(cr, wr0, iwr0);
+  double pst = fmax(((wr0 * pl + wl0 * pr) + wl0 * wr0 * (ul - ur)) * rcp_nr(wl0 + wr0), 0.0);
+
+  // Newton iteration (riemann.c:160-201)
+  const double g6 = ph.gamma6_d;
+  const double ipl = rcp2(pl), ipr = rcp2(pr);
+  const double gpl = g6 * ipl, gpr = g6 * ipr;
+  for (int it = 0; it < ph.niter; it++) {
+    double wwl, iwl, wwr, iwr;
+    sqrt_rsq_nr(cl * (1.0 + (pst - pl) * gpl), wwl, iwl);
+    sqrt_rsq_nr(cr * (1.0 + (pst - pr) * gpr), wwr, iwr);
+    const double swl = wwl * wwl, swr = wwr * wwr;
+    const double al = 2.0 * wwl * swl, bl = swl + cl;
+    const double ar = 2.0 * wwr * swr, br = swr + cr;
+    const double usl = ul - (pst - pl) * iwl;
+    const double usr = ur + (pst - pr) * iwr;
+    const double t = div_nr(al * ar, al * br + ar * bl) * (usl - usr);
+    const double d = (t > (-pst)) ? t : (-pst);
+    pst += d;
+    if (!(fabs(d) > 1e-6 * (pst + ph.smallpp_d))) break;
+  }
+
+  // star state and sampling at x/t = 0 (riemann.c:211-276)
+  const double ps = pst;
+  double wl, iwl, wr, iwr;
+  sqrt_rsq2(cr * (one + (ps - pr) * gpr), wr, iwr);
+  sqrt_rsq2(cl * (one + (ps - pl) * gpl), wl, iwl);
+  // ((ul + (pl-ps)/wl) + ur) - (pr-ps)/wr, the reference's left-to-right sum
+  const double ustar = half * (((ul + (pl - ps) * iwl) + ur) - (pr - ps) * iwr);
+  const bool left = ustar > 0.0;
+  const double sg = left ? 1.0 : -1.0;
+  const double ro = left ? rl : rr;
+  const double uo = left ? ul : ur;
+  const double po = left ? pl : pr;
+  const double wo = left ? wl : wr;
+  const double iwo = left ? iwl : iwr;
+  const double wo0 = left ? wl0 : wr0;
+  const double iro = rcp2(ro);
+  double rstar = ro * rcp2(one + ro * (po - ps) * (iwo * iwo));
+  rstar = fmax(rstar, ph.smallr);
+  double co = wo0 * iro;  // sqrt(gamma po / ro)
+  double cstar = sqrt_1ulp(ph.gamma * ps * rcp2(rstar));
+  co = fmax(ph.smallc, co);
+  cstar = fmax(ph.smallc, cstar);
+  double spout = co - sg * uo;
+  double spin = cstar - sg * ustar;
+  const double ushock = wo * iro - sg * uo;
+  if (ps >= po) {
+    spin = ushock;
+    spout = ushock;
+  }
+  const double scr = fmax(spout - spin, ph.smallc + fabs(spout + spin));
+  double frac = (one + (spout + spin) * rcp2(scr)) * half;
+  frac = fmax(0.0, fmin(one, frac));
+  double gr, gu, gp;
+  if (spout < 0.0) {
+    gr = ro;
+    gu = uo;
+    gp = po;
+  } else if (spin > 0.0) {
+    gr = rstar;
+    gu = ustar;
+    gp = ps;
+  } else {
+    gr = frac * rstar + (one - frac) * ro;
+    gu = frac * ustar + (one - frac) * uo;
+    gp = frac * ps + (one - frac) * po;
+  }
+  const double gv = left ? ql.v : qr.v;
+  Q4<double> f;
+  const double m = gr * gu;
+  f.r = m;
+  f.u = m * gu + gp;
+  f.v = m * gv;
+  const double ekin = half * gr * (gu * gu + gv * gv);
+  const double etot = gp * ph.entho + ekin;
+  f.p = gu * (etot + gp);
+  return f;
+}
+
 // The mixed-precision form (R = float) of riemann_flux: FP32 inputs, first
 // guess and star state, the Newton iteration in FP64 with the refined
 // hardware estimates (div_nr, rcp_nr, sqrt_rsq_nr).
@@ -714,12 +922,15 @@ __device__ __forceinline__ Q4<R> riemann_flux_mixed(const Q4<R>& ql, const Q4<R>
 }
 
 // riemann.c:128-276 for one face (qleft = ql, qright = qr) followed by
-// cmpflx.c:67-90: returns the Godunov flux through the face.  FD (FP64): the
-// range-gated divisions may be used; without it the IEEE code alone.
-template <typename R, bool FD = false>
+// cmpflx.c:67-90: returns the Godunov flux through the face.  FD (FP64, the
+// form): kGated -- the range-gated divisions may be used; kIeee -- the IEEE
+// code alone; kFast -- riemann_fast.
+template <typename R, int FD = kIeee>
 __device__ __forceinline__ Q4<R> riemann_flux(const Q4<R>& ql, const Q4<R>& qr,
                                               const Phys<R>& ph) {
-  if constexpr (sizeof(R) == 8) {
+  if constexpr (sizeof(R) == 8 && FD == kFast) {
+    return riemann_fast(ql, qr, ph);
+  } else if constexpr (sizeof(R) == 8) {
     const double rl = fmax(ql.r, ph.smallr);
     const double ul = ql.u;
     const double pl = fmax(ql.p, rl * ph.smallp);
@@ -728,7 +939,7 @@ __device__ __forceinline__ Q4<R> riemann_flux(const Q4<R>& ql, const Q4<R>& qr,
     const double pr = fmax(qr.p, rr * ph.smallp);
     const double cl = ph.gamma * pl * rl;
     const double cr = ph.gamma * pr * rr;
-    if constexpr (FD) {
+    if constexpr (FD == kGated) {
       // the face's floored densities and pressures in_pos, velocities in_zero
       const bool fast = ph.fastdiv && wave_all(in_pos(rl) && in_pos(pl) && in_pos(rr) &&
                                                in_pos(pr) && in_zero(ul) && in_zero(ur));

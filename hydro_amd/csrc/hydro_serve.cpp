@@ -7,9 +7,17 @@
 // (new arrivals join the runs already on the GPU), and while that runs copies
 // out the results of the requests whose time loop ended (t >= tend or nstep
 // >= nstepmax) in the previous pump -- on a second stream, so the copies
-// overlap the kernels -- and returns their memory to the budget.  A request
-// whose dt stops being a positive finite number is failed and retired on its
-// own; the others go on.
+// overlap the kernels -- and returns their memory to the budget.
+//
+// Requests that can never run (parameters a batch rejects, a precision other
+// than the server's, a state larger than the whole budget) fail at submission,
+// so they never hold up the queue.  A request whose run stops being finite
+// (its dt not a positive finite number, or a non-finite cell in its final
+// state) is, on a mixed-precision server, run again from its initial state in
+// FP64 -- on a second batch created on first use, within the same budget, at
+// the head of the queue -- since the FP32 total energy of the mixed form can
+// lose the pressure of hypersonic cells; if it breaks down in FP64 too (or on
+// an FP64 server) it fails alone and the others go on.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -18,6 +26,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -28,6 +37,7 @@ int hydro_amd_batch_download_raw(hydro_amd_batch* b, int id, void* dst, int* ele
                                  hipStream_t st);  // hydro_batch.cpp
 int hydro_amd_batch_raw_view(hydro_amd_batch* b, int id, const void** src, long* pitch,
                              long* width, long* rows, int* esz);
+int hydro_amd_batch_finite(hydro_amd_batch* b, const int* ids, int n, int* finite);
 
 namespace {
 enum { QUEUED = 0, RUNNING = 1, DONE = 2, FAILED = 3 };
@@ -77,6 +87,8 @@ struct Request {
   double t_submit = 0, t_admit = 0, t_done = 0;
   int state = QUEUED;
   int inst = -1;
+  int lane = 0;  // batch: 0 the server's, 1 the FP64 re-run batch
+  std::vector<double> u0;  // initial state when given whole (else the blast)
   double t_sim = 0, dt = 0;
   int nstep = 0;
   std::unique_ptr<unsigned char[]> result;  // state as stored (8- or 4-byte elements)
@@ -96,17 +108,21 @@ double now_s() {
 }  // namespace
 
 struct hydro_amd_server {
-  hydro_amd_batch* b = nullptr;
-  hipStream_t copy_st = nullptr;  // result copies, beside the batch's kernels
+  // b[0]: the server's precision; b[1]: FP64 re-runs of mixed-precision runs
+  // that broke down (created on first use)
+  hydro_amd_batch* b[2] = {nullptr, nullptr};
+  int prec = HYDRO_AMD_MIXED;
+  long long budget = 0;           // over both batches
+  hipStream_t copy_st = nullptr;  // result copies, beside the batches' kernels
   int chunk = 20;
   int keep = 0;
   std::vector<Request> req;
   std::deque<int> queue;
-  std::map<int, int> running;   // instance id -> request id
-  std::map<int, int> finished;  // ended on the device, result not yet copied out
-  std::vector<std::pair<int, int>> copying;  // (instance, request): DMA into the arena queued
+  std::map<int, int> running[2];   // per batch: instance id -> request id
+  std::map<int, int> finished[2];  // ended on the device, result not yet copied out
+  std::vector<std::tuple<int, int, int>> copying;  // (batch, instance, request): DMA queued
   Arena arena;
-  int ndone = 0, nfailed = 0;
+  int ndone = 0, nfailed = 0, nrerun = 0;
 };
 
 extern "C" double hydro_amd_now(void) { return now_s(); }
@@ -120,7 +136,9 @@ extern "C" hydro_amd_server* hydro_amd_server_create(int precision, long long bu
   hydro_amd_batch* b = hydro_amd_batch_create(precision, budget_bytes);
   if (!b) return nullptr;
   hydro_amd_server* s = new hydro_amd_server();
-  s->b = b;
+  s->b[0] = b;
+  s->prec = precision;
+  s->budget = budget_bytes > 0 ? budget_bytes : (1LL << 62);
   if (hipStreamCreateWithFlags(&s->copy_st, hipStreamNonBlocking) != hipSuccess) {
     hydro_amd_set_error("stream creation failed");
     hydro_amd_batch_destroy(b);
@@ -135,7 +153,8 @@ extern "C" hydro_amd_server* hydro_amd_server_create(int precision, long long bu
 extern "C" void hydro_amd_server_destroy(hydro_amd_server* s) {
   if (!s) return;
   if (s->copy_st) (void) hipStreamSynchronize(s->copy_st);
-  hydro_amd_batch_destroy(s->b);
+  hydro_amd_batch_destroy(s->b[0]);
+  hydro_amd_batch_destroy(s->b[1]);
   if (s->copy_st) (void) hipStreamDestroy(s->copy_st);
   if (s->arena.base) (void) hipHostFree(s->arena.base);
   delete s;
@@ -154,7 +173,42 @@ extern "C" int hydro_amd_server_reserve_results(hydro_amd_server* s, long long b
   return 0;
 }
 
-extern "C" hydro_amd_batch* hydro_amd_server_batch(hydro_amd_server* s) { return s ? s->b : nullptr; }
+extern "C" hydro_amd_batch* hydro_amd_server_batch(hydro_amd_server* s) {
+  return s ? s->b[0] : nullptr;
+}
+
+static long long used_bytes(const hydro_amd_server* s) {
+  long long u = hydro_amd_batch_bytes_used(s->b[0]);
+  if (s->b[1]) u += hydro_amd_batch_bytes_used(s->b[1]);
+  return u;
+}
+
+// Device bytes a run of p holds in the given precision (state buffers; the
+// same for every batch of that precision).
+static long long request_bytes(const hydro_amd_server* s, const hydro_amd_params& p, int prec) {
+  const long long n = hydro_amd_batch_bytes_needed(s->b[0], p.nx, p.ny);
+  if (n < 0) return -1;
+  const int have = s->prec == HYDRO_AMD_MIXED ? 4 : 8, want = prec == HYDRO_AMD_MIXED ? 4 : 8;
+  return n / have * want;
+}
+
+// Why a request can never run (nullptr if it can): what hydro_amd_batch_add
+// rejects whatever the load, or a state larger than the whole budget.
+static const char* never_runs(const hydro_amd_server* s, const hydro_amd_params& p) {
+  if (p.precision != s->prec) return "precision differs from the server's";
+  if (!(p.dx > 0) || !(p.gamma > 1) || p.scheme < 1 || p.scheme > 3 || p.niter_riemann < 0)
+    return "invalid run parameters";
+  if ((long) ((p.nx + 59) / 60) * p.ny > (1L << 30)) return "grid too large";
+  if (request_bytes(s, p, s->prec) > s->budget) return "state larger than the whole budget";
+  return nullptr;
+}
+
+static void fail_request(hydro_amd_server* s, Request& r) {
+  r.state = FAILED;
+  r.t_done = now_s();
+  r.inst = -1;
+  s->nfailed++;
+}
 
 extern "C" int hydro_amd_server_submit(hydro_amd_server* s, const hydro_amd_params* p, int bx,
                                        int by, double e) {
@@ -169,7 +223,19 @@ extern "C" int hydro_amd_server_submit(hydro_amd_server* s, const hydro_amd_para
   r.t_submit = now_s();
   s->req.push_back(std::move(r));
   const int id = (int) s->req.size() - 1;
-  s->queue.push_back(id);
+  if (never_runs(s, *p)) fail_request(s, s->req[id]);  // at once: it would stall the queue
+  else s->queue.push_back(id);
+  return id;
+}
+
+extern "C" int hydro_amd_server_submit_state(hydro_amd_server* s, const hydro_amd_params* p,
+                                             const double* uold) {
+  if (!s || !p || !uold) return hydro_amd_set_error("null argument");
+  if (p->nx < 1 || p->ny < 1) return hydro_amd_set_error("invalid request geometry");
+  const int id = hydro_amd_server_submit(s, p, 0, 0, 0.0);
+  if (id < 0) return -1;
+  Request& r = s->req[id];
+  r.u0.assign(uold, uold + (size_t) 4 * (p->nx + 4) * (p->ny + 4));
   return id;
 }
 
@@ -178,12 +244,13 @@ extern "C" int hydro_amd_server_submit(hydro_amd_server* s, const hydro_amd_para
 // has drained), 0 if not.  The instance is inactive on the device (its run
 // control ended), so kernels queued meanwhile on the batch's stream leave its
 // state alone.
-static int start_copy(hydro_amd_server* s, int inst, int rid) {
+static int start_copy(hydro_amd_server* s, int lane, int inst, int rid) {
   Request& r = s->req[rid];
   const void* src;
   long pitch, width, rows;
   int esz;
-  if (!s->arena.base || hydro_amd_batch_raw_view(s->b, inst, &src, &pitch, &width, &rows, &esz))
+  if (!s->arena.base ||
+      hydro_amd_batch_raw_view(s->b[lane], inst, &src, &pitch, &width, &rows, &esz))
     return 0;
   const size_t bytes = (size_t) width * rows * esz;
   size_t off;
@@ -203,15 +270,17 @@ static int start_copy(hydro_amd_server* s, int inst, int rid) {
 // Completes a request: copies its result out (staged, on the copy stream)
 // unless already in the arena, and frees its instance in the batch stream's
 // order.
-static int retire(hydro_amd_server* s, int inst, int rid, int state) {
+static int retire(hydro_amd_server* s, int lane, int inst, int rid, int state) {
   Request& r = s->req[rid];
   if (s->keep && state == DONE && r.arena_off == ~(size_t) 0) {
     const size_t n = (size_t) 4 * (r.p.nx + 4) * (r.p.ny + 4);
-    r.result_bytes = n * (r.p.precision == HYDRO_AMD_FP64 ? 8 : 4);
+    r.result_bytes = n * (lane == 0 && s->prec == HYDRO_AMD_MIXED ? 4 : 8);
     r.result.reset(new unsigned char[r.result_bytes]);
-    if (hydro_amd_batch_download_raw(s->b, inst, r.result.get(), &r.esz, s->copy_st)) return -1;
+    if (hydro_amd_batch_download_raw(s->b[lane], inst, r.result.get(), &r.esz, s->copy_st))
+      return -1;
   }
-  if (hydro_amd_batch_remove(s->b, inst)) return -1;
+  if (hydro_amd_batch_remove(s->b[lane], inst)) return -1;
+  std::vector<double>().swap(r.u0);
   r.state = state;
   r.t_done = now_s();
   r.inst = -1;
@@ -220,73 +289,85 @@ static int retire(hydro_amd_server* s, int inst, int rid, int state) {
   return 0;
 }
 
-// One scheduling round; returns the number of requests still queued,
-// running or waiting for their result to be copied out (0 when the server is
-// idle), or -1 on error.
-// Admits queued requests in arrival order while the head fits the budget.
+// A mixed-precision run that broke down: drop its instance and queue it again
+// at the head, from its initial state, for the FP64 batch (if its FP64 state
+// fits the budget; otherwise it fails).
+static int rerun_fp64(hydro_amd_server* s, int inst, int rid) {
+  Request& r = s->req[rid];
+  if (hydro_amd_batch_remove(s->b[0], inst)) return -1;
+  r.inst = -1;
+  hydro_amd_params p = r.p;
+  p.precision = HYDRO_AMD_FP64;
+  if (request_bytes(s, p, HYDRO_AMD_FP64) > s->budget) {
+    fail_request(s, r);
+    return 0;
+  }
+  r.lane = 1;
+  r.state = QUEUED;
+  r.t_sim = 0;
+  r.nstep = 0;
+  s->nrerun++;
+  s->queue.push_front(rid);
+  return 0;
+}
+
+static bool busy(const hydro_amd_server* s) {
+  for (int l = 0; l < 2; l++)
+    if (!s->running[l].empty() || !s->finished[l].empty()) return true;
+  return !s->copying.empty();
+}
+
+// Admits queued requests in arrival order (re-runs first) while the head
+// fits the budget.
 static int admit(hydro_amd_server* s) {
   while (!s->queue.empty()) {
     const int rid = s->queue.front();
     Request& r = s->req[rid];
-    const int inst = hydro_amd_batch_add(s->b, &r.p);
+    hydro_amd_params p = r.p;
+    if (r.lane == 1) {
+      p.precision = HYDRO_AMD_FP64;
+      if (!s->b[1] && !(s->b[1] = hydro_amd_batch_create(HYDRO_AMD_FP64, s->budget))) return -1;
+    }
+    const long long need = request_bytes(s, p, p.precision);
+    if (used_bytes(s) + need > s->budget) break;  // fits once running requests retire
+    const int inst = hydro_amd_batch_add(s->b[r.lane], &p);
     if (inst < 0) {
-      if (s->running.empty() && s->finished.empty()) {  // can never fit: fail, don't stall
+      if (!busy(s)) {  // nothing else holds memory: it will not fit later either
         s->queue.pop_front();
-        r.state = FAILED;
-        r.t_done = now_s();
-        s->nfailed++;
+        fail_request(s, r);
         continue;
       }
       break;
     }
-    if (hydro_amd_batch_init_blast(s->b, inst, r.bx, r.by, r.e)) return -1;
+    if (r.u0.empty() ? hydro_amd_batch_init_blast(s->b[r.lane], inst, r.bx, r.by, r.e)
+                     : hydro_amd_batch_upload(s->b[r.lane], inst, r.u0.data()))
+      return -1;
     s->queue.pop_front();
     r.state = RUNNING;
     r.inst = inst;
     r.t_admit = now_s();
-    s->running[inst] = rid;
+    s->running[r.lane][inst] = rid;
   }
   return 0;
 }
 
-extern "C" int hydro_amd_server_pump(hydro_amd_server* s) {
-  if (!s) return hydro_amd_set_error("null server");
-  if (admit(s)) return -1;
-  // queue the next chunk, then copy out the runs that ended in the last one
-  // while it executes
-  if (!s->running.empty() && hydro_amd_batch_run_async(s->b, s->chunk)) return -1;
-  bool freed = false;
-  for (const auto& f : s->finished) {
-    const int q = s->keep ? start_copy(s, f.first, f.second) : 0;
-    if (q < 0) return -1;
-    if (q) {
-      s->copying.emplace_back(f.first, f.second);
-    } else {
-      if (retire(s, f.first, f.second, DONE)) return -1;
-      freed = true;
-    }
-  }
-  s->finished.clear();
-  // requests that fit in the memory just returned join from the next chunk
-  if (freed && admit(s)) return -1;
-  if (!s->running.empty() && hydro_amd_batch_sync(s->b)) return -1;
-  if (!s->copying.empty()) {
-    if (hipStreamSynchronize(s->copy_st) != hipSuccess)
-      return hydro_amd_set_error("result copy failed");
-    for (const auto& c : s->copying)
-      if (retire(s, c.first, c.second, DONE)) return -1;
-    s->copying.clear();
-  }
-  if (s->running.empty()) return (int) s->queue.size();
-  const int n = (int) s->running.size();
+// Reads the run state of a batch's running requests: ended runs move to
+// `finished` (a mixed run whose final state is not finite is run again in
+// FP64), runs that broke down are re-run (mixed) or fail.
+static int poll(hydro_amd_server* s, int lane) {
+  auto& run = s->running[lane];
+  if (run.empty()) return 0;
+  const int n = (int) run.size();
   std::vector<int> ids(n), nstep(n), more(n);
   std::vector<double> t(n), dt(n);
-  const int got = hydro_amd_batch_query(s->b, ids.data(), t.data(), nstep.data(), dt.data(),
-                                        more.data(), n);
+  const int got = hydro_amd_batch_query(s->b[lane], ids.data(), t.data(), nstep.data(),
+                                        dt.data(), more.data(), n);
   if (got < 0) return -1;
+  const bool can_rerun = lane == 0 && s->prec == HYDRO_AMD_MIXED;
+  std::vector<int> ended;
   for (int k = 0; k < got && k < n; k++) {
-    auto it = s->running.find(ids[k]);
-    if (it == s->running.end()) continue;
+    auto it = run.find(ids[k]);
+    if (it == run.end()) continue;
     const int rid = it->second;
     Request& r = s->req[rid];
     r.t_sim = t[k];
@@ -294,21 +375,83 @@ extern "C" int hydro_amd_server_pump(hydro_amd_server* s) {
     r.dt = dt[k];
     const bool sane = more[k] >= 0 && isfinite(t[k]);
     if (!sane) {
-      if (retire(s, ids[k], rid, FAILED)) return -1;
-      s->running.erase(it);
+      run.erase(it);
+      if (can_rerun) {
+        if (rerun_fp64(s, ids[k], rid)) return -1;
+      } else if (retire(s, lane, ids[k], rid, FAILED)) {
+        return -1;
+      }
     } else if (more[k] == 0) {
-      s->finished[ids[k]] = rid;
-      s->running.erase(it);
+      ended.push_back(ids[k]);
     }
   }
-  return (int) (s->queue.size() + s->running.size() + s->finished.size());
+  if (ended.empty()) return 0;
+  // a breakdown in the last steps leaves a non-finite state without a bad dt
+  std::vector<int> fin(ended.size(), 1);
+  if (can_rerun && hydro_amd_batch_finite(s->b[lane], ended.data(), (int) ended.size(),
+                                          fin.data()))
+    return -1;
+  for (size_t k = 0; k < ended.size(); k++) {
+    const int rid = run[ended[k]];
+    run.erase(ended[k]);
+    if (fin[k]) {
+      s->finished[lane][ended[k]] = rid;
+    } else if (rerun_fp64(s, ended[k], rid)) {
+      return -1;
+    }
+  }
+  return 0;
+}
+
+// One scheduling round; returns the number of requests still queued,
+// running or waiting for their result to be copied out (0 when the server is
+// idle), or -1 on error.
+extern "C" int hydro_amd_server_pump(hydro_amd_server* s) {
+  if (!s) return hydro_amd_set_error("null server");
+  if (admit(s)) return -1;
+  // queue the next chunk, then copy out the runs that ended in the last one
+  // while it executes
+  for (int l = 0; l < 2; l++)
+    if (!s->running[l].empty() && hydro_amd_batch_run_async(s->b[l], s->chunk)) return -1;
+  bool freed = false;
+  for (int l = 0; l < 2; l++) {
+    for (const auto& f : s->finished[l]) {
+      const int q = s->keep ? start_copy(s, l, f.first, f.second) : 0;
+      if (q < 0) return -1;
+      if (q) {
+        s->copying.emplace_back(l, f.first, f.second);
+      } else {
+        if (retire(s, l, f.first, f.second, DONE)) return -1;
+        freed = true;
+      }
+    }
+    s->finished[l].clear();
+  }
+  // requests that fit in the memory just returned join from the next chunk
+  if (freed && admit(s)) return -1;
+  for (int l = 0; l < 2; l++)
+    if (!s->running[l].empty() && hydro_amd_batch_sync(s->b[l])) return -1;
+  if (!s->copying.empty()) {
+    if (hipStreamSynchronize(s->copy_st) != hipSuccess)
+      return hydro_amd_set_error("result copy failed");
+    for (const auto& c : s->copying)
+      if (retire(s, std::get<0>(c), std::get<1>(c), std::get<2>(c), DONE)) return -1;
+    s->copying.clear();
+  }
+  for (int l = 0; l < 2; l++)
+    if (poll(s, l)) return -1;
+  size_t left = s->queue.size();
+  for (int l = 0; l < 2; l++) left += s->running[l].size() + s->finished[l].size();
+  return (int) left;
 }
 
 extern "C" int hydro_amd_server_counts(hydro_amd_server* s, int* queued, int* running, int* done,
                                        int* failed) {
   if (!s) return hydro_amd_set_error("null server");
   if (queued) *queued = (int) s->queue.size();
-  if (running) *running = (int) (s->running.size() + s->finished.size());
+  if (running)
+    *running = (int) (s->running[0].size() + s->finished[0].size() + s->running[1].size() +
+                      s->finished[1].size());
   if (done) *done = s->ndone;
   if (failed) *failed = s->nfailed;
   return 0;
@@ -355,4 +498,15 @@ extern "C" int hydro_amd_server_release(hydro_amd_server* s, int rid) {
   r.result.reset();
   r.result_bytes = 0;
   return 0;
+}
+
+// The precision a request's result was computed in: the server's, or FP64
+// for a mixed-precision run that was run again after breaking down.
+extern "C" int hydro_amd_server_result_precision(hydro_amd_server* s, int rid) {
+  if (!s || rid < 0 || rid >= (int) s->req.size()) return hydro_amd_set_error("no request %d", rid);
+  return s->req[rid].lane == 1 ? HYDRO_AMD_FP64 : s->prec;
+}
+
+extern "C" int hydro_amd_server_reruns(hydro_amd_server* s) {
+  return s ? s->nrerun : hydro_amd_set_error("null server");
 }

@@ -137,9 +137,10 @@ struct SegWalk {
   }
 };
 
-// FD: the range-gated FP64 divisions may be used (FP64 launches with enough
-// waves to keep the SIMDs busy); without it the IEEE code alone.
-template <typename S, bool ST1, bool COUR, bool FD>
+// FD, the arithmetic form (hydro_numerics.hpp): kGated -- the range-gated
+// FP64 divisions may be used (FP64 launches with enough waves to keep the
+// SIMDs busy); kIeee -- the IEEE code alone; kFast -- the FP64 fast form.
+template <typename S, bool ST1, bool COUR, int FD>
 __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
                                           const TraceK<S>& tk, double dtdx,
                                           const GridView<S>& in, const GridView<S>& out,
@@ -151,17 +152,23 @@ __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
   const long o = j * g.pitch + ic;
 
   const S U0 = in.p[0][o], U1 = in.p[1][o], U2 = in.p[2][o], U3 = in.p[3][o];
-  // every cell of the wave (the slope neighbours included) in range: FastDiv
-  const bool fc = FD && wave_cell_in<S>(U0, U1, U2, U3, ph);
-  const Prim<S> pc = fc ? prim_cell<S, true>(U0, U1, U2, U3, ph)
-                        : prim_cell<S, false>(U0, U1, U2, U3, ph);
+  Prim<S> pc;
+  bool fc = false;
+  if constexpr (FD == kFast) {
+    pc = prim_fast(U0, U1, U2, U3, ph);
+  } else {
+    // every cell of the wave (the slope neighbours included) in range: FastDiv
+    fc = FD == kGated && wave_cell_in<S>(U0, U1, U2, U3, ph);
+    pc = fc ? prim_cell<S, true>(U0, U1, U2, U3, ph) : prim_cell<S, false>(U0, U1, U2, U3, ph);
+  }
   Q4<S> dq = {(S) 0, (S) 0, (S) 0, (S) 0};
   if (ph.iorder != 1) {
     const Q4<S> qm = from_left4(pc.q), qp = from_right4(pc.q);
     dq = slope4<S, ST1>(qm, pc.q, qp, ph.slope_type);
   }
   Q4<S> qxm, qxp;
-  if (fc) trace_cell<S, true, true>(pc.q, dq, pc.c, tk, qxm, qxp, pc.rinv);
+  if constexpr (FD == kFast) trace_fast(pc.q, dq, pc.c, pc.cinv, pc.rinv, tk, qxm, qxp);
+  else if (fc) trace_cell<S, true, true>(pc.q, dq, pc.c, tk, qxm, qxp, pc.rinv);
   else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);  // (fc is false in FP32)
   const Q4<S> qr = from_right4(qxp);
   // flux through the face between this lane's cell and the next one
@@ -226,7 +233,7 @@ __device__ __forceinline__ Q4<S> lds_get(const S (&b)[4][64 * W + 2], int slot) 
 // once per wave.  Every thread of the workgroup calls it with the same
 // (jrow, seg): it has barriers.  Consecutive calls may reuse `sh`: each
 // buffer is rewritten only after a barrier that follows its last read.
-template <typename S, bool ST1, bool COUR, int W, bool FD>
+template <typename S, bool ST1, bool COUR, int W, int FD>
 __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
                                         const TraceK<S>& tk, double dtdx,
                                         const GridView<S>& in, const GridView<S>& out,
@@ -240,20 +247,26 @@ __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
   const long o = j * g.pitch + ic;
 
   const S U0 = in.p[0][o], U1 = in.p[1][o], U2 = in.p[2][o], U3 = in.p[3][o];
-  // FastDiv for the primitives when the wave's cells are in range, for the
-  // traces when every wave's are (the slopes read neighbours of other waves)
-  const bool fc = FD && wave_cell_in<S>(U0, U1, U2, U3, ph);
-  const Prim<S> pc = fc ? prim_cell<S, true>(U0, U1, U2, U3, ph)
-                        : prim_cell<S, false>(U0, U1, U2, U3, ph);
+  // kGated: FastDiv for the primitives when the wave's cells are in range,
+  // for the traces when every wave's are (the slopes read neighbours of
+  // other waves)
+  Prim<S> pc;
+  bool fc = false;
+  if constexpr (FD == kFast) {
+    pc = prim_fast(U0, U1, U2, U3, ph);
+  } else {
+    fc = FD == kGated && wave_cell_in<S>(U0, U1, U2, U3, ph);
+    pc = fc ? prim_cell<S, true>(U0, U1, U2, U3, ph) : prim_cell<S, false>(U0, U1, U2, U3, ph);
+  }
   Q4<S> dq = {(S) 0, (S) 0, (S) 0, (S) 0};
   bool ft = fc;
   if (ph.iorder != 1) {  // launch-uniform
     lds_put<W>(sh.q, t, pc.q);
-    if constexpr (FD) {
+    if constexpr (FD == kGated) {
       if ((t & 63) == 0) sh.in[t >> 6] = fc;
     }
     __syncthreads();
-    if constexpr (FD) {
+    if constexpr (FD == kGated) {
       int all = 1;
       for (int w = 0; w < W; w++) all &= sh.in[w];
       ft = __builtin_amdgcn_readfirstlane(all) != 0;  // the same in every lane
@@ -261,7 +274,9 @@ __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
     dq = slope4<S, ST1>(lds_get<W>(sh.q, t), pc.q, lds_get<W>(sh.q, t + 2), ph.slope_type);
   }
   Q4<S> qxm, qxp;
-  if constexpr (FD) {
+  if constexpr (FD == kFast) {
+    trace_fast(pc.q, dq, pc.c, pc.cinv, pc.rinv, tk, qxm, qxp);
+  } else if constexpr (FD == kGated) {
     if (ft) trace_cell<S, true, true>(pc.q, dq, pc.c, tk, qxm, qxp, pc.rinv);  // ft implies fc
     else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);
   } else {
@@ -346,7 +361,7 @@ __device__ __forceinline__ Cell<S> load_y(const GridView<S>& in, long o) {
   return Cell<S>{in.p[0][o], in.p[2][o], in.p[1][o], in.p[3][o]};
 }
 
-template <typename S, bool ST1, bool COUR, bool FD>
+template <typename S, bool ST1, bool COUR, int FD>
 __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
                                         const TraceK<S>& tk, double dtdx,
                                         const GridView<S>& in, const GridView<S>& out,
@@ -365,30 +380,34 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
   // rows j0 and j0+1 await their updates
   stash_put_cell(ys->u[j0 % 3], lane, uc);
   stash_put_cell(ys->u[(j0 + 1) % 3], lane, ud);
-  // FastDiv per row when the wave's cells of the row are in range (a trace
-  // also needs the rows above and below it in range)
+  // kGated: FastDiv per row when the wave's cells of the row are in range (a
+  // trace also needs the rows above and below it in range)
   auto prim = [&](const Cell<S>& u, bool f) {
-    return f ? prim_cell<S, true>(u.u0, u.u1, u.u2, u.u3, ph)
-             : prim_cell<S, false>(u.u0, u.u1, u.u2, u.u3, ph);
+    if constexpr (FD == kFast) return prim_fast(u.u0, u.u1, u.u2, u.u3, ph);
+    else
+      return f ? prim_cell<S, true>(u.u0, u.u1, u.u2, u.u3, ph)
+               : prim_cell<S, false>(u.u0, u.u1, u.u2, u.u3, ph);
   };
-  auto trace = [&](const Q4<S>& q, const Q4<S>& dq, S c, bool f, Q4<S>& qxm, Q4<S>& qxp) {
-    if (f) trace_cell<S, true>(q, dq, c, tk, qxm, qxp);
-    else trace_cell<S, false>(q, dq, c, tk, qxm, qxp);
+  auto trace = [&](const Prim<S>& pq, const Q4<S>& dq, bool f, Q4<S>& qxm, Q4<S>& qxp) {
+    if constexpr (FD == kFast) trace_fast(pq.q, dq, pq.c, pq.cinv, pq.rinv, tk, qxm, qxp);
+    else if (f) trace_cell<S, true>(pq.q, dq, pq.c, tk, qxm, qxp);
+    else trace_cell<S, false>(pq.q, dq, pq.c, tk, qxm, qxp);
   };
-  const bool fa = FD && wave_cell_in<S>(ua.u0, ua.u1, ua.u2, ua.u3, ph);
-  const bool fb = FD && wave_cell_in<S>(ub.u0, ub.u1, ub.u2, ub.u3, ph);
-  bool fC = FD && wave_cell_in<S>(uc.u0, uc.u1, uc.u2, uc.u3, ph);
-  bool fD = FD && wave_cell_in<S>(ud.u0, ud.u1, ud.u2, ud.u3, ph);
+  auto cell_ok = [&](const Cell<S>& u) {
+    return FD == kGated && wave_cell_in<S>(u.u0, u.u1, u.u2, u.u3, ph);
+  };
+  const bool fa = cell_ok(ua);
+  const bool fb = cell_ok(ub);
+  bool fC = cell_ok(uc);
+  bool fD = cell_ok(ud);
   Prim<S> pa = prim(ua, fa);
   Prim<S> pb = prim(ub, fb);
   Prim<S> pcc = prim(uc, fC);
   Prim<S> pd = prim(ud, fD);
   const Q4<S> z = {(S) 0, (S) 0, (S) 0, (S) 0};
   Q4<S> qxm_b, qxp_b, qxm_c, qxp_c;
-  trace(pb.q, o1 ? z : slope4<S, ST1>(pa.q, pb.q, pcc.q, st), pb.c, fa && fb && fC, qxm_b,
-        qxp_b);
-  trace(pcc.q, o1 ? z : slope4<S, ST1>(pb.q, pcc.q, pd.q, st), pcc.c, fb && fC && fD, qxm_c,
-        qxp_c);
+  trace(pb, o1 ? z : slope4<S, ST1>(pa.q, pb.q, pcc.q, st), fa && fb && fC, qxm_b, qxp_b);
+  trace(pcc, o1 ? z : slope4<S, ST1>(pb.q, pcc.q, pd.q, st), fb && fC && fD, qxm_c, qxp_c);
   stash_put(ys->f, lane, to_d4(riemann_flux<S, FD>(qxm_b, qxp_c, ph)));  // face (j0-1 | j0)
   stash_put(ys->xm, lane, qxm_c);
 
@@ -396,7 +415,7 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
   // qxm of C, the flux through the face below C and the conserved values of
   // C and D
   Q4<S> qC = pcc.q, qD = pd.q;
-  S cD = pd.c, rD = pd.rinv;
+  S cD = pd.c, rD = pd.rinv, iD = pd.cinv;
   // the next row's loads are issued one row ahead of their use
   Cell<S> uN = load_y(in, (long) (j0 + 2) * P + i);
   // One row of the march: take row j, prefetch row j+1, update row j-2,
@@ -411,12 +430,13 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
     stash_put_cell(ys->u[sj], lane, uE);
     const int jn = j + 1 <= j1 + 1 ? j + 1 : j1 + 1;  // last row: a harmless reload
     uN = load_y(in, (long) jn * P + i);
-    const bool fE = FD && wave_cell_in<S>(uE.u0, uE.u1, uE.u2, uE.u3, ph);
+    const bool fE = cell_ok(uE);
     const Prim<S> pE = prim(uE, fE);
     Q4<S> qxmD, qxpD;
     {
       const Q4<S> dqD = o1 ? z : slope4<S, ST1>(qC, qD, pE.q, st);
-      if (fC && fD && fE) trace_cell<S, true, true>(qD, dqD, cD, tk, qxmD, qxpD, rD);
+      if constexpr (FD == kFast) trace_fast(qD, dqD, cD, iD, rD, tk, qxmD, qxpD);
+      else if (fC && fD && fE) trace_cell<S, true, true>(qD, dqD, cD, tk, qxmD, qxpD, rD);
       else trace_cell<S, false>(qD, dqD, cD, tk, qxmD, qxpD);
     }
     const Q4<S> qxmC = stash_get(ys->xm, lane);
@@ -449,6 +469,7 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
     qD = pE.q;
     cD = pE.c;
     rD = pE.rinv;
+    iD = pE.cinv;
     fC = fD;
     fD = fE;
   };

@@ -24,6 +24,13 @@ extern "C" {
 #endif
 
 #define HYDRO_AMD_FP64 0   /* FP64 storage and arithmetic (bitwise the reference's) */
+#define HYDRO_AMD_FP64_FAST 2 /* FP64 storage and arithmetic, not bit for bit: fused
+                              multiply-adds, quotients and square roots from the
+                              hardware estimates refined to about an ulp, and the
+                              Newton iteration with one division per step.  Agrees
+                              with FP64 to a per-field relative L1 of 1e-12 on
+                              finite flows (smallr, smallc > 0); the fastest FP64
+                              form. */
 #define HYDRO_AMD_MIXED 1  /* FP32 storage / slopes / traces / fluxes; FP64 Newton,
                               update accumulation and CFL reduction.  The total
                               energy is stored in FP32, so a cell whose internal
@@ -160,12 +167,17 @@ int hydro_amd_batch_autotune(hydro_amd_batch *b, int reps, int *best);
 
 /* ---- serving: continuous batching of independent runs --------------------
  * Requests queue in arrival order.  Each pump admits queued requests while
- * they fit the HBM budget (FIFO; a request larger than the whole budget
- * fails), advances every running request by chunk_steps steps in one batched
- * device run -- requests admitted now join those already on the GPU -- and
- * retires the runs whose time loop has ended (t >= tend or nstep >=
- * nstepmax), copying results out when keep_results and returning their
- * memory.  A run whose dt stops being positive and finite is failed alone.
+ * they fit the HBM budget (FIFO), advances every running request by
+ * chunk_steps steps in one batched device run -- requests admitted now join
+ * those already on the GPU -- and retires the runs whose time loop has ended
+ * (t >= tend or nstep >= nstepmax), copying results out when keep_results and
+ * returning their memory.  A request that can never run (invalid parameters,
+ * a precision other than the server's, a state larger than the whole budget)
+ * is failed at submission, so it never holds up the queue.  A run whose dt
+ * stops being positive and finite (or whose final state is not finite) is,
+ * on a mixed-precision server, run again from its initial state in FP64 at
+ * the head of the queue; otherwise, or if it breaks down in FP64 too, it is
+ * failed alone.
  * Timestamps use hydro_amd_now() (CLOCK_MONOTONIC seconds). */
 typedef struct hydro_amd_server hydro_amd_server;
 double hydro_amd_now(void);
@@ -177,6 +189,10 @@ hydro_amd_batch *hydro_amd_server_batch(hydro_amd_server *s);
  * (energy e at interior cell bx, by); returns the request id. */
 int hydro_amd_server_submit(hydro_amd_server *s, const hydro_amd_params *p, int bx, int by,
                             double e);
+/* Queue a run of p from a whole initial state (host, the reference layout
+ * [4][ny+4][nx+4] FP64; copied at submission, uploaded at admission). */
+int hydro_amd_server_submit_state(hydro_amd_server *s, const hydro_amd_params *p,
+                                  const double *uold);
 /* One round; returns requests still queued or running (0 = idle), -1 error. */
 int hydro_amd_server_pump(hydro_amd_server *s);
 int hydro_amd_server_counts(hydro_amd_server *s, int *queued, int *running, int *done,
@@ -186,6 +202,12 @@ int hydro_amd_server_counts(hydro_amd_server *s, int *queued, int *running, int 
 int hydro_amd_server_info(hydro_amd_server *s, int req, double *times, double *t_sim, int *nstep);
 int hydro_amd_server_result(hydro_amd_server *s, int req, double *uold);
 int hydro_amd_server_release(hydro_amd_server *s, int req);
+/* The precision a finished request's result was computed in: the server's,
+ * or HYDRO_AMD_FP64 for a mixed-precision run that broke down (its dt not a
+ * positive finite number, or a non-finite cell at its end) and was run again
+ * from its initial state in FP64.  reruns: how many such re-runs there were. */
+int hydro_amd_server_result_precision(hydro_amd_server *s, int req);
+int hydro_amd_server_reruns(hydro_amd_server *s);
 /* Pin `bytes` of host memory for kept results (once per server): a result
  * that fits is copied by DMA straight into it while the next chunk runs;
  * others are copied through staging buffers.  release() returns the space. */

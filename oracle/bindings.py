@@ -11,7 +11,9 @@ import numpy as np
 
 ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
 PORT_SO = os.path.join(ORACLE_DIR, "_port", "libhydro_oracle.so")
+PORT_FMA_SO = os.path.join(ORACLE_DIR, "_port", "libhydro_oracle_fma.so")
 REF_SO = os.path.join(ORACLE_DIR, "_ref", "libhydroref.so")
+REF_SP_SO = os.path.join(ORACLE_DIR, "_ref", "libhydroref_sp.so")  # -DPREC_SP
 REF_OMP = os.path.join(ORACLE_DIR, "_ref", "hydro_omp")
 REF_D = os.path.join(ORACLE_DIR, "_ref", "hydro_d")
 
@@ -111,12 +113,16 @@ def write_deck(p: Params, path: str | None = None) -> str:
 
 
 class Port:
-    """This repository's C99 restatement (oracle/hydro_oracle.c)."""
+    """This repository's C99 restatement (oracle/hydro_oracle.c).  fma=True
+    loads its build with FMA contraction: an equally valid FP64 rounding of
+    the same algorithm, used to measure how far rounding alone moves a result
+    (not bit-identical to the reference)."""
 
-    def __init__(self):
-        if not port_available():
+    def __init__(self, fma: bool = False):
+        path = PORT_FMA_SO if fma else PORT_SO
+        if not os.path.exists(path):
             build(ref=False)
-        lib = C.CDLL(PORT_SO)
+        lib = C.CDLL(path)
         P = C.POINTER(_HoParams)
         lib.ho_init.argtypes = [P, C.c_int, _dp]
         lib.ho_init_blast.argtypes = [P, C.c_int, C.c_int, C.c_double, _dp]
@@ -232,15 +238,19 @@ class Ref:
     """The reference HydroC99_2DMpi sources, compiled in place (oracle/_ref).
 
     The reference keeps its state in globals, so one deck is open per process
-    at a time; ``open``/``close`` bracket it."""
+    at a time (per build); ``open``/``close`` bracket it.  sp=True loads the
+    reference's all-FP32 build (-DPREC_SP: real_t = float,
+    HydroC99_2DMpi/Src/parametres.h:41-45); uold, dt and t cross as FP64
+    either way, and its stage functions are not bound."""
 
-    _lib = None
+    _libs = {}
 
-    def __init__(self):
-        if not ref_available():
-            raise FileNotFoundError(REF_SO + " not built (needs the reference sources)")
-        if Ref._lib is None:
-            lib = C.CDLL(REF_SO)
+    def __init__(self, sp: bool = False):
+        path = REF_SP_SO if sp else REF_SO
+        if not os.path.exists(path):
+            raise FileNotFoundError(path + " not built (needs the reference sources)")
+        if path not in Ref._libs:
+            lib = C.CDLL(path)
             lib.ref_open.argtypes = [C.c_char_p, _ip]
             lib.ref_open.restype = C.c_int
             lib.ref_get_reals.argtypes = [_dp]
@@ -255,25 +265,33 @@ class Ref:
             lib.ref_steps.argtypes = [C.c_int, _dp, _dp]
             lib.ref_steps.restype = C.c_int
             lib.ref_vtkfile.argtypes = [C.c_int]
-            # the reference's own stage functions (HydroC99_2DMpi/Src/*.h), used
-            # with slices = Hstep = 1 so the work arrays are SoA [nvar][Hnxyt]
-            lib.constoprim.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
-                                       _dp, _dp, _dp]
-            lib.equation_of_state.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
-                                              C.c_double, C.c_int, C.c_int, _dp, _dp, _dp]
-            lib.slope.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _dp,
-                                  _dp]
-            lib.trace.argtypes = [C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                  C.c_int, _dp, _dp, _dp, _dp, _dp]
-            lib.qleftright.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                       C.c_int, _dp, _dp, _dp, _dp]
-            lib.cmpflx.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _dp,
-                                   _dp]
-            lib.ref_riemann_line.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
-                                             C.c_double, C.c_int, _dp, _dp, _dp, _ip]
-            Ref._lib = lib
-        self.lib = Ref._lib
+            if sp:
+                Ref._libs[path] = lib
+        if path not in Ref._libs:
+            lib = Ref._lib_stages(lib)
+            Ref._libs[path] = lib
+        self.lib = Ref._libs[path]
         self.dims = None
+
+    @staticmethod
+    def _lib_stages(lib):
+        # the reference's own stage functions (HydroC99_2DMpi/Src/*.h), used
+        # with slices = Hstep = 1 so the work arrays are SoA [nvar][Hnxyt]
+        lib.constoprim.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                   _dp, _dp, _dp]
+        lib.equation_of_state.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                          C.c_double, C.c_int, C.c_int, _dp, _dp, _dp]
+        lib.slope.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _dp,
+                              _dp]
+        lib.trace.argtypes = [C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                              C.c_int, _dp, _dp, _dp, _dp, _dp]
+        lib.qleftright.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, _dp, _dp, _dp, _dp]
+        lib.cmpflx.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _dp,
+                               _dp]
+        lib.ref_riemann_line.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
+                                         C.c_double, C.c_int, _dp, _dp, _dp, _ip]
+        return lib
 
     # time loop on a deck
     def open(self, p: Params, gamma: float | None = None):

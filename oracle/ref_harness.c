@@ -77,19 +77,31 @@ void ref_get_ints(int *v) {
 /* gamma is hard-wired in the reference's default_values(); let tests vary it. */
 void ref_set_gamma(double g) { H.gamma = g; }
 
+/* uold crosses as FP64 whatever real_t is: the build with -DPREC_SP (the
+ * reference's all-FP32 mode, parametres.h:41-45) converts on the way. */
 long ref_uold_len(void) { return (long) H.nvar * H.nxt * H.nyt; }
-void ref_get_uold(double *dst) { memcpy(dst, Hv.uold, sizeof(double) * ref_uold_len()); }
-void ref_set_uold(const double *src) { memcpy(Hv.uold, src, sizeof(double) * ref_uold_len()); }
+void ref_get_uold(double *dst) {
+  for (long k = 0; k < ref_uold_len(); k++) dst[k] = Hv.uold[k];
+}
+void ref_set_uold(const double *src) {
+  for (long k = 0; k < ref_uold_len(); k++) Hv.uold[k] = (real_t) src[k];
+}
 
-void ref_compute_deltat(double *dt) { compute_deltat(dt, H, &Hw_deltat, &Hv, &Hvw_deltat); }
-void ref_godunov(int idim, double dt) { hydro_godunov(idim, dt, H, &Hv, &Hw_godunov, &Hvw_godunov); }
+void ref_compute_deltat(double *dt) {
+  real_t d = 0;
+  compute_deltat(&d, H, &Hw_deltat, &Hv, &Hvw_deltat);
+  *dt = d;
+}
+void ref_godunov(int idim, double dt) {
+  hydro_godunov(idim, (real_t) dt, H, &Hv, &Hw_godunov, &Hvw_godunov);
+}
 void ref_make_boundary(int idim) { make_boundary(idim, H, &Hv); }
 void ref_vtkfile(int step) { vtkfile(step, H, &Hv); }
 
 /* Runs up to k steps of the reference time loop; returns the steps taken.
  * dt_log[s] / t_log[s] receive dt used by, and t after, each step. */
 int ref_steps(int k, double *dt_log, double *t_log) {
-  static double dt = 0;
+  static real_t dt = 0;  /* main.c:144 */
   int done = 0;
   while ((H.t < H.tend) && (H.nstep < H.nstepmax) && done < k) {
     if ((H.nstep % 2) == 0) {
@@ -110,6 +122,7 @@ int ref_steps(int k, double *dt_log, double *t_log) {
   return done;
 }
 
+#ifndef PREC_SP
 /* Calls the reference riemann() on one line of nface faces (SoA [4][nl]
  * arrays), allocating the hydrowork_t scratch it expects. */
 #include "riemann.h"
@@ -127,6 +140,7 @@ void ref_riemann_line(int nface, int nl, double smallr, double smallc, double ga
   free(w.goon);
   free(blk);
 }
+#endif
 
 /* The reference's process_args alone (no allocation): reals as ref_get_reals,
  * ints: nx, ny, nxystep, nstepmax, testcase, niter_riemann, iorder, scheme,

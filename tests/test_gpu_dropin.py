@@ -42,7 +42,7 @@ def vtk_files(d):
     return sorted(out)
 
 
-@pytest.mark.parametrize("testcase", [0, 1])
+@pytest.mark.parametrize("testcase", [0, 1, 2])
 def test_reference_driver_on_gpu_library(tmp_path, testcase):
     p = oracle.Params(nx=120, ny=70, dx=1 / 120.0, courant_factor=0.8, tend=1e30, nstepmax=40,
                       testcase=testcase)

@@ -115,3 +115,95 @@ def test_mixed_precision_error_vs_fp64():
             a, b = um[v, 2:-2, 2:-2], u64[v, 2:-2, 2:-2]
             l1 = np.abs(a - b).sum() / np.abs(b).sum()
             assert l1 <= tol, (r.params.nx, v, l1)
+
+
+def test_requests_that_can_never_run_fail_at_once():
+    """Requests a batch would always refuse -- another precision, invalid
+    parameters, a state larger than the whole budget -- placed ahead of good
+    ones fail at submission, and the good ones behind them are served without
+    waiting for the queue to drain."""
+    runs = small_runs(4, 8, hydro_amd.MIXED, lo=32, hi=64)
+    with hydro_amd.Batch(hydro_amd.MIXED) as b:
+        budget = 4 * b.bytes_needed(64, 64)
+    base = runs[0].params
+    bad = [hydro_amd.Params(**{**base.__dict__, "precision": hydro_amd.FP64}),
+           hydro_amd.Params(**{**base.__dict__, "gamma": 0.9}),
+           hydro_amd.Params(**{**base.__dict__, "scheme": 7}),
+           hydro_amd.Params(**{**base.__dict__, "nx": 1024, "ny": 1024, "dx": 1 / 1024.0})]
+    with hydro_amd.Server(precision=hydro_amd.MIXED, budget_bytes=budget, chunk_steps=10,
+                          keep_results=True) as s:
+        bids = [s.submit(p, 1, 1, 100.0) for p in bad]
+        for i in bids:
+            assert s.info(i)[0] == s.FAILED  # before any pump
+        ids = [s.submit(r.params, r.bx, r.by, r.e) for r in runs]
+        s.pump()
+        assert s.counts()["running"] > 0  # the good ones were admitted at the first pump
+        while s.pump() > 0:
+            pass
+        for rid, run in zip(ids, runs):
+            assert s.info(rid)[0] == s.DONE
+            assert bits_differ(s.result(rid), alone(run)[0]) == 0
+        assert s.counts()["failed"] == len(bad)
+
+
+def hypersonic_state(seed):
+    """A state of the random-configuration draws with cells whose kinetic
+    energy dwarfs their internal energy (velocities to +-8, pressures down to
+    1e-3): the FP32 total energy of the mixed form loses their pressure."""
+    from test_gpu_random_configs import draw
+    p, u, steps = draw(seed)
+    return p, u, steps
+
+
+def test_mixed_breakdown_is_run_again_in_fp64():
+    """Mixed-precision requests that break down -- a blast whose fluxes overflow
+    FP32 (E = 1e35) and a hypersonic random state on which the mixed form goes
+    non-finite -- end DONE, run again in FP64, with results bit for bit those
+    of the FP64 run alone; an ordinary request beside them stays mixed."""
+    blast = hydro_amd.Params(nx=48, ny=40, dx=1 / 48.0, courant_factor=0.8, tend=1e30,
+                             nstepmax=30, precision=hydro_amd.MIXED)
+    # a random-configuration state on which the mixed form breaks down alone
+    hyper = None
+    for seed in range(20000, 20200):
+        p, u0, steps = hypersonic_state(seed)
+        if p.nx * p.ny > 300 * 300:
+            continue
+        p.nstepmax, p.tend = steps + 20, 1e30
+        p64 = hydro_amd.Params(**{**p.__dict__, "precision": hydro_amd.FP64})
+        with hydro_amd.Grid(p64) as g:
+            g.upload(u0)
+            g.run(p.nstepmax, log=False)
+            u64 = g.download()
+        if not np.isfinite(u64).all():
+            continue
+        p.precision = hydro_amd.MIXED
+        with hydro_amd.Grid(p) as g:
+            g.upload(u0)
+            g.run(p.nstepmax, log=False)
+            um = g.download()
+        if not np.isfinite(um).all():
+            hyper = (p, u0, u64)
+            break
+    assert hyper is not None, "no random state broke the mixed form"
+    ok = small_runs(1, 11, hydro_amd.MIXED)[0]
+    with hydro_amd.Server(precision=hydro_amd.MIXED, budget_bytes=10**9, chunk_steps=8,
+                          keep_results=True) as s:
+        rb = s.submit(blast, 20, 10, 1e35)
+        rh = s.submit_state(hyper[0], hyper[1])
+        ro = s.submit(ok.params, ok.bx, ok.by, ok.e)
+        while s.pump() > 0:
+            pass
+        assert s.info(rb)[0] == s.DONE and s.info(rh)[0] == s.DONE and s.info(ro)[0] == s.DONE
+        assert s.result_precision(rb) == hydro_amd.FP64
+        assert s.result_precision(rh) == hydro_amd.FP64
+        assert s.result_precision(ro) == hydro_amd.MIXED
+        assert s.reruns == 2
+        b64 = hydro_amd.Params(**{**blast.__dict__, "precision": hydro_amd.FP64})
+        with hydro_amd.Grid(b64) as g:
+            g.init_blast(20, 10, 1e35)
+            g.run(10**6, log=False)
+            ub = g.download()
+        assert np.isfinite(ub).all()
+        assert bits_differ(s.result(rb), ub) == 0
+        assert bits_differ(s.result(rh), hyper[2]) == 0
+        assert bits_differ(s.result(ro), alone(ok)[0]) == 0
