@@ -382,6 +382,25 @@ def pmc_workload(nx, precision, steps, tune=None):
                       "nx": nx, "steps": steps}), flush=True)
 
 
+def pmc_batch_workload(precision, n, steps):
+    """A short fixed workload for counter and trace runs of the batched
+    kernels: the 1 GiB copy kernel, then `steps` steps of the n-instance
+    batch (sides log-uniform in [64, 1024]) with the batch's default task
+    shape."""
+    import hydro_amd
+    from hydro_amd import workloads as W
+    gbs = hydro_amd.copy_probe(1 << 30, 1)
+    runs = W.batch_instances(n, steps=steps, precision=precision)
+    with hydro_amd.Batch(precision) as b:
+        ids = [b.add(r.params) for r in runs]
+        for i, r in zip(ids, runs):
+            b.init_blast(i, r.bx, r.by, r.e)
+        b.run(steps)
+    print(json.dumps({"pmc_batch_workload": True, "copy_bytes": 1 << 30, "copy_GBps": gbs,
+                      "instances": n, "steps": steps,
+                      "cells": sum(r.params.nx * r.params.ny for r in runs)}), flush=True)
+
+
 COPY_PROBE_BYTES = 1 << 30  # each of read and written by pmc_workload's copy kernel
 
 
@@ -466,6 +485,8 @@ def measured_valu(nx, precision, tune, kern, avg_s):
     cycles = v["GRBM_GUI_ACTIVE"] / XCDS
     return {"kernel": kern,
             "valu_instr_per_cell": round(64 * v["SQ_INSTS_VALU"] / cells, 1),
+            "fp64_instr_per_cell": {c[len("SQ_INSTS_VALU_"):].lower(): round(64 * v[c] / cells, 1)
+                                    for c in VALU_COUNTERS[1:5]},
             "fp64_flops_per_cell": round(flops / cells, 1),
             "fp64_tflops": round(flops / avg_s / 1e12, 2),
             "fp64_peak_tflops": FP64_VALU_PEAK / 1e12,
@@ -498,6 +519,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--pmc-workload", action="store_true",
                     help="short fixed run for rocprofv3 counter passes")
+    ap.add_argument("--pmc-batch-workload", action="store_true",
+                    help="short fixed batched run (--batch-n instances, --steps steps) for "
+                         "rocprofv3 counter and trace passes")
     ap.add_argument("--tune", default="", help="yseg,ywaves,xwaves,xiter (pmc workload)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter passes behind roofline.traffic and valu")
@@ -511,6 +535,9 @@ def main():
     if a.pmc_workload:
         pmc_workload(a.nx, prec, a.steps,
                      [int(v) for v in a.tune.split(",")] if a.tune else None)
+        return
+    if a.pmc_batch_workload:
+        pmc_batch_workload(prec, a.batch_n, a.steps)
         return
     r = gpu_bench(a.nx, ny, a.steps, a.warmup, prec, autotune=not a.no_autotune,
                   testcase=a.testcase)

@@ -39,6 +39,8 @@ constexpr int kFast = 2;
 template <typename R>
 struct Phys {
   R gamma, gm1, smallr, smallc, smallp, entho, slope_type;
+  R slope_inv;    // 1 / slope_type (the FP64 fast form's slopes)
+  double smallr_inv;  // 1 / smallr (the FP64 fast form's star state)
   double gamma_d, gamma6_d, smallp_d, smallpp_d;  // Newton constants (double)
   int niter;      // niter_riemann
   int iorder;     // 1: first order (dq = 0)
@@ -59,6 +61,8 @@ __host__ __device__ inline Phys<R> make_phys(double gamma, double smallr, double
   p.smallp = ((R) smallc * (R) smallc) / (R) gamma;
   p.entho = (R) 1 / ((R) gamma - (R) 1);
   p.slope_type = (R) slope_type;
+  p.slope_inv = (R) (1.0 / slope_type);
+  p.smallr_inv = 1.0 / smallr;
   p.gamma_d = gamma;
   p.gamma6_d = (gamma + 1.0) / (2.0 * gamma);
   p.smallp_d = (smallc * smallc) / gamma;
@@ -349,12 +353,15 @@ __device__ __forceinline__ bool wave_cell_in(S U0, S U1, S U2, S U3, const Phys<
 //
 // Quotients and square roots from the hardware estimates v_rcp_f64 and
 // v_rsq_f64 (relative error about 2^-24, measured on MI355X) refined by
-// Newton-Raphson or coupled Goldschmidt steps instead of the correctly rounded sequences: two steps
-// (rcp2, sqrt_rsq2, sqrt_1ulp) leave a result within about an ulp, and
-// divisions by one denominator share its reciprocal.  Inside the Newton
-// pressure iteration one step (rcp_nr, sqrt_rsq_nr: about 2^-48) is enough:
-// an error in an iterate is damped by the next Newton step, so p* carries
-// only the error of its last correction, which is below 1e-6 of p*.
+// Newton-Raphson or coupled Goldschmidt steps instead of the correctly
+// rounded sequences: two steps (rcp2, sqrt_rsq2, sqrt_1ulp) leave a result
+// within about an ulp, and divisions by one denominator share its
+// reciprocal.  Inside the Newton pressure iteration one step (rcp_nr,
+// sqrt_rsq_nr: about 2^-48) is enough except in the last iteration the
+// limit allows: an error in an iterate is damped by the next Newton step, so
+// p* carries only the error of its last correction, below 1e-6 of p* when
+// the face converged, and a face stopped by the limit takes that last step
+// with the two-step estimates.
 
 // 1/b within about an ulp: the estimate and two Newton-Raphson steps.
 __device__ __forceinline__ double rcp2(double b) {
@@ -391,13 +398,15 @@ __device__ __forceinline__ double sqrt_1ulp(double x) {
 __device__ __forceinline__ Prim<double> prim_fast(double U0, double U1, double U2, double U3,
                                                   const Phys<double>& ph) {
   Prim<double> o;
-  const double r = maxm(U0, ph.smallr);
+  // (fmax: one v_max_f64; the same values as the reference's MAX here, whose
+  // second operand is never NaN)
+  const double r = fmax(U0, ph.smallr);
   const double ir = rcp2(r);
   const double u = U1 * ir, v = U2 * ir;
   const double eken = 0.5 * (u * u + v * v);
   const double e = U3 * ir - eken;
   double p = ph.gm1 * r * e;
-  p = maxm(p, r * ph.smallp);
+  p = fmax(p, r * ph.smallp);
   sqrt_rsq2(ph.gamma * p * ir, o.c, o.cinv);  // gamma p / rho > 0
   o.q = Q4<double>{r, u, v, p};
   o.rinv = ir;
@@ -444,6 +453,37 @@ __device__ __forceinline__ R slope1(R qm, R q0, R qp, R st) {
   return sg * fmin(keep * t1, fabs(dc));
 }
 
+// slope.c:71-90 in the FP64 fast form: the same limiter with the sign
+// copied from dc (a zero slope keeps the sign of a zero dc, where the
+// reference's sg = -1 gives -0) and, for slope_type != 1, dc scaled by
+// 1/slope_type.
+template <bool ST1>
+__device__ __forceinline__ double slope1_fast(double qm, double q0, double qp, double st,
+                                              double ist) {
+  double dl, dr, dc;
+  if (ST1) {
+    dl = q0 - qm;
+    dr = qp - q0;
+    dc = 0.5 * (dl + dr);
+  } else {
+    dl = st * (q0 - qm);
+    dr = st * (qp - q0);
+    dc = (0.5 * (dl + dr)) * ist;
+  }
+  const double m = (dl * dr <= 0.0) ? 0.0 : fmin(fmin(fabs(dl), fabs(dr)), fabs(dc));
+  return copysign(m, dc);
+}
+
+template <bool ST1>
+__device__ __forceinline__ Q4<double> slope4_fast(const Q4<double>& qm, const Q4<double>& q0,
+                                                  const Q4<double>& qp, const Phys<double>& ph) {
+  const double st = ph.slope_type, ist = ph.slope_inv;
+  return Q4<double>{slope1_fast<ST1>(qm.r, q0.r, qp.r, st, ist),
+                    slope1_fast<ST1>(qm.u, q0.u, qp.u, st, ist),
+                    slope1_fast<ST1>(qm.v, q0.v, qp.v, st, ist),
+                    slope1_fast<ST1>(qm.p, q0.p, qp.p, st, ist)};
+}
+
 template <typename R, bool ST1>
 __device__ __forceinline__ Q4<R> slope4(const Q4<R>& qm, const Q4<R>& q0, const Q4<R>& qp,
                                         R st) {
@@ -453,6 +493,14 @@ __device__ __forceinline__ Q4<R> slope4(const Q4<R>& qm, const Q4<R>& q0, const 
   d.v = slope1<R, ST1>(qm.v, q0.v, qp.v, st);
   d.p = slope1<R, ST1>(qm.p, q0.p, qp.p, st);
   return d;
+}
+
+// The limited slopes of a cell in form FD (slope4, or slope4_fast for kFast).
+template <typename R, bool ST1, int FD>
+__device__ __forceinline__ Q4<R> slope_form(const Q4<R>& qm, const Q4<R>& q0, const Q4<R>& qp,
+                                            const Phys<R>& ph) {
+  if constexpr (FD == kFast && sizeof(R) == 8) return slope4_fast<ST1>(qm, q0, qp, ph);
+  else return slope4<R, ST1>(qm, q0, qp, ph.slope_type);
 }
 
 // Trace constants of trace.c:63-80.
@@ -493,12 +541,25 @@ __device__ __forceinline__ void trace_tail(const Q4<R>& q, const Q4<R>& dq, R cc
   R a0v = dq.v;
   R upcc = q.u + cc;
   R umcc = q.u - cc;
+  // The predictor factors (s * dtdx +- one: one multiply-add where the unit
+  // contracts, the reference's product then sum in the FP64 unit), clamped to
+  // +-project where s has crossed zeror / zerol.  With umcc <= u <= upcc no
+  // clamp applies on a lane with upcc < zeror and umcc > zerol, so the
+  // selects run only when some lane of the wave fails that test (every lane
+  // under schemes 2 and 3, where zeror = zerol = 0; a lane whose test fails
+  // for a NaN takes the selects, which keep the unclamped NaN).
+  R spmR = umcc * k.dtdx + one, sppR = upcc * k.dtdx + one, sp0R = q.u * k.dtdx + one;
+  R spmL = umcc * k.dtdx - one, sppL = upcc * k.dtdx - one, sp0L = q.u * k.dtdx - one;
+  if (!wave_all(upcc < k.zeror && umcc > k.zerol)) {
+    spmR = (umcc >= k.zeror) ? k.project : spmR;
+    sppR = (upcc >= k.zeror) ? k.project : sppR;
+    sp0R = (q.u >= k.zeror) ? k.project : sp0R;
+    spmL = (umcc <= k.zerol) ? -k.project : spmL;
+    sppL = (upcc <= k.zerol) ? -k.project : sppL;
+    sp0L = (q.u <= k.zerol) ? -k.project : sp0L;
+  }
   {
-    // (umcc * dtdx + one: one multiply-add where the unit contracts, the
-    // reference's product then sum in the FP64 unit)
-    R spm = (umcc >= k.zeror) ? k.project : umcc * k.dtdx + one;
-    R spp = (upcc >= k.zeror) ? k.project : upcc * k.dtdx + one;
-    R sp0 = (q.u >= k.zeror) ? k.project : q.u * k.dtdx + one;
+    const R spm = spmR, spp = sppR, sp0 = sp0R;
     R apr = -half * spp * ap;
     R amr = -half * spm * am;
     R azr = -half * sp0 * a0r;
@@ -509,9 +570,7 @@ __device__ __forceinline__ void trace_tail(const Q4<R>& q, const Q4<R>& dq, R cc
     qxp.p = q.p + (apr + amr) * csq;
   }
   {
-    R spm = (umcc <= k.zerol) ? -k.project : umcc * k.dtdx - one;
-    R spp = (upcc <= k.zerol) ? -k.project : upcc * k.dtdx - one;
-    R sp0 = (q.u <= k.zerol) ? -k.project : q.u * k.dtdx - one;
+    const R spm = spmL, spp = sppL, sp0 = sp0L;
     R apl = -half * spp * ap;
     R aml = -half * spm * am;
     R azl = -half * sp0 * a0r;
@@ -697,13 +756,48 @@ __device__ __forceinline__ Q4<double> riemann_d(const Q4<double>& ql, const Q4<d
                             IeeeDiv::R{pr}, ph);
 }
 
-// trace.c:85-147, fast form: the four quotients from 1/rho and 1/c.
+// trace.c:85-147, fast form: the four quotients from 1/rho and 1/c, and the
+// predictor's products regrouped: -half * sp * (half * x * rOcc) is
+// sp * (x * (-rho / 4c)), and each face's sums fold into multiply-adds.
 __device__ __forceinline__ void trace_fast(const Q4<double>& q, const Q4<double>& dq, double cc,
                                            double ic, double ir, const TraceK<double>& k,
                                            Q4<double>& qxm, Q4<double>& qxp) {
+  const double csq = cc * cc;
   const double dpic = dq.p * ic;
-  trace_tail<double>(q, dq, cc, cc * cc, q.r * ic, cc * ir, dpic * ir, dq.r - dpic * ic, k, qxm,
-                     qxp);
+  const double dprcc = dpic * ir;          // dp / (rho c)
+  const double a0r = dq.r - dpic * ic;     // drho - dp / c^2
+  const double OrOcc = cc * ir;            // c / rho
+  const double qr4 = -0.25 * (q.r * ic);   // -rho / (4c)
+  const double nap = (dprcc + dq.u) * qr4;  // -ap / 2
+  const double nam = (dprcc - dq.u) * qr4;  // -am / 2
+  const double na0 = -0.5 * a0r, nav = -0.5 * dq.v;
+  const double upcc = q.u + cc, umcc = q.u - cc;
+  double spmR = umcc * k.dtdx + 1.0, sppR = upcc * k.dtdx + 1.0, sp0R = q.u * k.dtdx + 1.0;
+  double spmL = umcc * k.dtdx - 1.0, sppL = upcc * k.dtdx - 1.0, sp0L = q.u * k.dtdx - 1.0;
+  if (!wave_all(upcc < k.zeror && umcc > k.zerol)) {  // as in trace_tail
+    spmR = (umcc >= k.zeror) ? k.project : spmR;
+    sppR = (upcc >= k.zeror) ? k.project : sppR;
+    sp0R = (q.u >= k.zeror) ? k.project : sp0R;
+    spmL = (umcc <= k.zerol) ? -k.project : spmL;
+    sppL = (upcc <= k.zerol) ? -k.project : sppL;
+    sp0L = (q.u <= k.zerol) ? -k.project : sp0L;
+  }
+  {
+    const double amr = spmR * nam;
+    const double t = sppR * nap + amr;  // apr + amr
+    qxp.r = q.r + (t + sp0R * na0);
+    qxp.u = q.u + (sppR * nap - amr) * OrOcc;
+    qxp.v = q.v + sp0R * nav;
+    qxp.p = q.p + t * csq;
+  }
+  {
+    const double aml = spmL * nam;
+    const double t = sppL * nap + aml;
+    qxm.r = q.r + (t + sp0L * na0);
+    qxm.u = q.u + (sppL * nap - aml) * OrOcc;
+    qxm.v = q.v + sp0L * nav;
+    qxm.p = q.p + t * csq;
+  }
 }
 
 // riemann.c:128-276 + cmpflx.c:67-90 for one face, fast form.  The first
@@ -713,8 +807,9 @@ __device__ __forceinline__ void trace_fast(const Q4<double>& q, const Q4<double>
 // q = a / b (a = 2 w^3, b = w^2 + c) qr ql / (qr + ql) = al ar / (al br +
 // ar bl) is one division, and |d / (p* + smallpp)| > 1e-6 is tested as
 // |d| > 1e-6 (p* + smallpp).  In the star state, sqrt(gamma po / ro) is
-// sqrt(gamma po ro) / ro with sqrt(gamma po ro) from the first guess, and
-// wo / ro, ro / (1 + ...) and (spout + spin) / scr are products with
+// sqrt(gamma po ro) / ro with sqrt(gamma po ro) from the first guess; with
+// rstar = ro / X, one reciprocal of ro X gives 1/ro, 1/X and 1/rstar (1/smallr
+// where the floor holds); wo / ro and (spout + spin) / scr are products with
 // reciprocals.
 __device__ __forceinline__ Q4<double> riemann_fast(const Q4<double>& ql, const Q4<double>& qr,
                                                    const Phys<double>& ph) {
@@ -733,23 +828,34 @@ __device__ __forceinline__ Q4<double> riemann_fast(const Q4<double>& ql, const Q
   double wl0, iwl0, wr0, iwr0;
   sqrt_rsq2(cl, wl0, iwl0);
   sqrt_rsq2(cr, wr0, iwr0);
-  double pst = fmax(((wr0 * pl + wl0 * pr) + wl0 * wr0 * (ul - ur)) * rcp_nr(wl0 + wr0), 0.0);
+  double pst = fmax(((wr0 * pl + wl0 * pr) + wl0 * wr0 * (ul - ur)) * rcp2(wl0 + wr0), 0.0);
 
-  // Newton iteration (riemann.c:160-201)
+  // Newton iteration (riemann.c:160-201).  The last iteration the limit
+  // allows takes the estimates refined to an ulp: when a face stops there
+  // unconverged, that step's correction is not damped by a later one.
   const double g6 = ph.gamma6_d;
   const double ipl = rcp2(pl), ipr = rcp2(pr);
   const double gpl = g6 * ipl, gpr = g6 * ipr;
   for (int it = 0; it < ph.niter; it++) {
+    const bool last = it + 1 == ph.niter;  // wave-uniform
+    const double xl = cl * (1.0 + (pst - pl) * gpl), xr = cr * (1.0 + (pst - pr) * gpr);
     double wwl, iwl, wwr, iwr;
-    sqrt_rsq_nr(cl * (1.0 + (pst - pl) * gpl), wwl, iwl);
-    sqrt_rsq_nr(cr * (1.0 + (pst - pr) * gpr), wwr, iwr);
+    if (!last) {
+      sqrt_rsq_nr(xl, wwl, iwl);
+      sqrt_rsq_nr(xr, wwr, iwr);
+    } else {
+      sqrt_rsq2(xl, wwl, iwl);
+      sqrt_rsq2(xr, wwr, iwr);
+    }
     const double swl = wwl * wwl, swr = wwr * wwr;
     const double al = 2.0 * wwl * swl, bl = swl + cl;
     const double ar = 2.0 * wwr * swr, br = swr + cr;
     const double usl = ul - (pst - pl) * iwl;
     const double usr = ur + (pst - pr) * iwr;
-    const double t = div_nr(al * ar, al * br + ar * bl) * (usl - usr);
-    const double d = (t > (-pst)) ? t : (-pst);
+    const double den = al * br + ar * bl;
+    const double rden = last ? rcp2(den) : rcp_nr(den);
+    const double t = ((al * ar) * rden) * (usl - usr);
+    const double d = fmax(t, -pst);  // (t > -pst ? t : -pst for every pst that is not NaN)
     pst += d;
     if (!(fabs(d) > 1e-6 * (pst + ph.smallpp_d))) break;
   }
@@ -769,11 +875,15 @@ __device__ __forceinline__ Q4<double> riemann_fast(const Q4<double>& ql, const Q
   const double wo = left ? wl : wr;
   const double iwo = left ? iwl : iwr;
   const double wo0 = left ? wl0 : wr0;
-  const double iro = rcp2(ro);
-  double rstar = ro * rcp2(one + ro * (po - ps) * (iwo * iwo));
-  rstar = fmax(rstar, ph.smallr);
+  // rstar = ro / X: one reciprocal of ro X gives 1/ro, 1/X and 1/rstar
+  const double X = one + ro * (po - ps) * (iwo * iwo);
+  const double y = rcp2(ro * X);
+  const double iro = X * y;
+  const double rs0 = ro * (ro * y);  // ro / X
+  const double rstar = fmax(rs0, ph.smallr);
+  const double irstar = rs0 > ph.smallr ? X * iro : ph.smallr_inv;
   double co = wo0 * iro;  // sqrt(gamma po / ro)
-  double cstar = sqrt_1ulp(ph.gamma * ps * rcp2(rstar));
+  double cstar = sqrt_1ulp(ph.gamma * ps * irstar);
   co = fmax(ph.smallc, co);
   cstar = fmax(ph.smallc, cstar);
   double spout = co - sg * uo;

@@ -164,7 +164,7 @@ __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
   Q4<S> dq = {(S) 0, (S) 0, (S) 0, (S) 0};
   if (ph.iorder != 1) {
     const Q4<S> qm = from_left4(pc.q), qp = from_right4(pc.q);
-    dq = slope4<S, ST1>(qm, pc.q, qp, ph.slope_type);
+    dq = slope_form<S, ST1, FD>(qm, pc.q, qp, ph);
   }
   Q4<S> qxm, qxp;
   if constexpr (FD == kFast) trace_fast(pc.q, dq, pc.c, pc.cinv, pc.rinv, tk, qxm, qxp);
@@ -271,7 +271,7 @@ __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
       for (int w = 0; w < W; w++) all &= sh.in[w];
       ft = __builtin_amdgcn_readfirstlane(all) != 0;  // the same in every lane
     }
-    dq = slope4<S, ST1>(lds_get<W>(sh.q, t), pc.q, lds_get<W>(sh.q, t + 2), ph.slope_type);
+    dq = slope_form<S, ST1, FD>(lds_get<W>(sh.q, t), pc.q, lds_get<W>(sh.q, t + 2), ph);
   }
   Q4<S> qxm, qxp;
   if constexpr (FD == kFast) {
@@ -369,7 +369,6 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
                                         double& cx, double& cy) {
   const long P = g.pitch;
   const int lane = threadIdx.x & 63;
-  const S st = ph.slope_type;
   const bool o1 = ph.iorder == 1;
 
   // prologue: cells j0-2 .. j0+1
@@ -406,8 +405,10 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
   Prim<S> pd = prim(ud, fD);
   const Q4<S> z = {(S) 0, (S) 0, (S) 0, (S) 0};
   Q4<S> qxm_b, qxp_b, qxm_c, qxp_c;
-  trace(pb, o1 ? z : slope4<S, ST1>(pa.q, pb.q, pcc.q, st), fa && fb && fC, qxm_b, qxp_b);
-  trace(pcc, o1 ? z : slope4<S, ST1>(pb.q, pcc.q, pd.q, st), fb && fC && fD, qxm_c, qxp_c);
+  trace(pb, o1 ? z : slope_form<S, ST1, FD>(pa.q, pb.q, pcc.q, ph), fa && fb && fC, qxm_b,
+        qxp_b);
+  trace(pcc, o1 ? z : slope_form<S, ST1, FD>(pb.q, pcc.q, pd.q, ph), fb && fC && fD, qxm_c,
+        qxp_c);
   stash_put(ys->f, lane, to_d4(riemann_flux<S, FD>(qxm_b, qxp_c, ph)));  // face (j0-1 | j0)
   stash_put(ys->xm, lane, qxm_c);
 
@@ -434,7 +435,7 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
     const Prim<S> pE = prim(uE, fE);
     Q4<S> qxmD, qxpD;
     {
-      const Q4<S> dqD = o1 ? z : slope4<S, ST1>(qC, qD, pE.q, st);
+      const Q4<S> dqD = o1 ? z : slope_form<S, ST1, FD>(qC, qD, pE.q, ph);
       if constexpr (FD == kFast) trace_fast(qD, dqD, cD, iD, rD, tk, qxmD, qxpD);
       else if (fC && fD && fE) trace_cell<S, true, true>(qD, dqD, cD, tk, qxmD, qxpD, rD);
       else trace_cell<S, false>(qD, dqD, cD, tk, qxmD, qxpD);
