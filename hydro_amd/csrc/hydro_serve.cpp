@@ -235,7 +235,7 @@ extern "C" int hydro_amd_server_submit_state(hydro_amd_server* s, const hydro_am
   const int id = hydro_amd_server_submit(s, p, 0, 0, 0.0);
   if (id < 0) return -1;
   Request& r = s->req[id];
-  r.u0.assign(uold, uold + (size_t) 4 * (p->nx + 4) * (p->ny + 4));
+  if (r.state == QUEUED) r.u0.assign(uold, uold + (size_t) 4 * (p->nx + 4) * (p->ny + 4));
   return id;
 }
 
@@ -475,6 +475,7 @@ extern "C" int hydro_amd_server_info(hydro_amd_server* s, int rid, double* times
 
 extern "C" int hydro_amd_server_result(hydro_amd_server* s, int rid, double* uold) {
   if (!s || rid < 0 || rid >= (int) s->req.size()) return hydro_amd_set_error("no request %d", rid);
+  if (!uold) return hydro_amd_set_error("null argument");
   Request& r = s->req[rid];
   const unsigned char* src = r.data(s->arena);
   if (r.state != DONE || !src) return hydro_amd_set_error("no result for %d", rid);
