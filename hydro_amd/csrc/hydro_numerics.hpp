@@ -432,12 +432,30 @@ __device__ __forceinline__ void courant_cell(R U0, R U1, R U2, R U3, const Phys<
   cy = o.c + fabs(o.q.v);
 }
 
+// slope.c:71-90 with slope_type == 1 as a median.  There dc = (dl + dr) / 2,
+// and where dl and dr have one sign fl(|dl + dr|) / 2 >= min(|dl|, |dr|)
+// (rounding is monotone), so the limiter sg * fmin(keep * min(|dl|, |dr|),
+// |dc|) is min(|dl|, |dr|) with their sign, and zero where their signs differ
+// or one is zero: median(dl, dr, 0).  The same value as the reference's
+// except for the sign of a zero slope, and where dl * dr underflows to 0
+// (both below ~1e-162 in FP64, ~4e-23 in FP32), where the reference's slope
+// is 0 and this one the smaller of the two.  Used by the mixed and fast forms
+// only; the FP64 form keeps the reference's operations.
+__device__ __forceinline__ float slope1_med(float dl, float dr) {
+  return __builtin_amdgcn_fmed3f(dl, dr, 0.0f);
+}
+__device__ __forceinline__ double slope1_med(double dl, double dr) {
+  return fmax(fmin(dl, dr), fmin(fmax(dl, dr), 0.0));
+}
+
 // slope.c:71-90 (non-OLDSTYLE limiter) for one variable.  ST1 specialises
-// slope_type == 1, where st*x == x and x/st == x exactly.
+// slope_type == 1, where st*x == x and x/st == x exactly (in FP32, the
+// median form above).
 template <typename R, bool ST1>
 __device__ __forceinline__ R slope1(R qm, R q0, R qp, R st) {
   const R half = (R) 0.5;
   R dl, dr, dc;
+  if constexpr (ST1 && sizeof(R) == 4) return slope1_med(q0 - qm, qp - q0);
   if (ST1) {
     dl = q0 - qm;
     dr = qp - q0;
@@ -453,25 +471,22 @@ __device__ __forceinline__ R slope1(R qm, R q0, R qp, R st) {
   return sg * fmin(keep * t1, fabs(dc));
 }
 
-// slope.c:71-90 in the FP64 fast form: the same limiter with the sign
-// copied from dc (a zero slope keeps the sign of a zero dc, where the
-// reference's sg = -1 gives -0) and, for slope_type != 1, dc scaled by
-// 1/slope_type.
+// slope.c:71-90 in the FP64 fast form: for slope_type 1 the median form
+// (slope1_med); otherwise the same limiter with the sign copied from dc (a
+// zero slope keeps the sign of a zero dc, where the reference's sg = -1 gives
+// -0) and dc scaled by 1/slope_type.
 template <bool ST1>
 __device__ __forceinline__ double slope1_fast(double qm, double q0, double qp, double st,
                                               double ist) {
-  double dl, dr, dc;
-  if (ST1) {
-    dl = q0 - qm;
-    dr = qp - q0;
-    dc = 0.5 * (dl + dr);
+  if constexpr (ST1) {
+    return slope1_med(q0 - qm, qp - q0);
   } else {
-    dl = st * (q0 - qm);
-    dr = st * (qp - q0);
-    dc = (0.5 * (dl + dr)) * ist;
+    const double dl = st * (q0 - qm);
+    const double dr = st * (qp - q0);
+    const double dc = (0.5 * (dl + dr)) * ist;
+    const double m = (dl * dr <= 0.0) ? 0.0 : fmin(fmin(fabs(dl), fabs(dr)), fabs(dc));
+    return copysign(m, dc);
   }
-  const double m = (dl * dr <= 0.0) ? 0.0 : fmin(fmin(fabs(dl), fabs(dr)), fabs(dc));
-  return copysign(m, dc);
 }
 
 template <bool ST1>
@@ -582,6 +597,10 @@ __device__ __forceinline__ void trace_tail(const Q4<R>& q, const Q4<R>& dq, R cc
   }
 }
 
+template <typename R>
+__device__ __forceinline__ void trace_fast(const Q4<R>& q, const Q4<R>& dq, R cc, R ic, R ir,
+                                           const TraceK<R>& k, Q4<R>& qxm, Q4<R>& qxp);
+
 // trace.c:85-147 for one cell (quotients as the form computes them, then
 // trace_tail).  FAST (FP64): FastDiv, for cells whose own and neighbour
 // values (the slope inputs) passed cell_in.  RINV: rinv is prim_cell's
@@ -600,13 +619,11 @@ __device__ __forceinline__ void trace_cell(const Q4<R>& q, const Q4<R>& dq, R cc
     dprcc = P::div(dq.p, q.r * cc);
     a0r = dq.r - P::div(dq.p, csq);
   } else {
-    // two reciprocals for the four quotients
-    const R rcc = __builtin_amdgcn_rcpf(cc);
-    const R rr = __builtin_amdgcn_rcpf(q.r);
-    rOcc = q.r * rcc;
-    OrOcc = cc * rr;
-    dprcc = (dq.p * rr) * rcc;
-    a0r = dq.r - (dq.p * rcc) * rcc;
+    // FP32 (mixed form): two reciprocals for the four quotients, then the
+    // fast form's regrouped predictor
+    trace_fast<R>(q, dq, cc, __builtin_amdgcn_rcpf(cc), __builtin_amdgcn_rcpf(q.r), k, qxm,
+                  qxp);
+    return;
   }
   trace_tail<R>(q, dq, cc, csq, rOcc, OrOcc, dprcc, a0r, k, qxm, qxp);
 }
@@ -756,24 +773,26 @@ __device__ __forceinline__ Q4<double> riemann_d(const Q4<double>& ql, const Q4<d
                             IeeeDiv::R{pr}, ph);
 }
 
-// trace.c:85-147, fast form: the four quotients from 1/rho and 1/c, and the
-// predictor's products regrouped: -half * sp * (half * x * rOcc) is
-// sp * (x * (-rho / 4c)), and each face's sums fold into multiply-adds.
-__device__ __forceinline__ void trace_fast(const Q4<double>& q, const Q4<double>& dq, double cc,
-                                           double ic, double ir, const TraceK<double>& k,
-                                           Q4<double>& qxm, Q4<double>& qxp) {
-  const double csq = cc * cc;
-  const double dpic = dq.p * ic;
-  const double dprcc = dpic * ir;          // dp / (rho c)
-  const double a0r = dq.r - dpic * ic;     // drho - dp / c^2
-  const double OrOcc = cc * ir;            // c / rho
-  const double qr4 = -0.25 * (q.r * ic);   // -rho / (4c)
-  const double nap = (dprcc + dq.u) * qr4;  // -ap / 2
-  const double nam = (dprcc - dq.u) * qr4;  // -am / 2
-  const double na0 = -0.5 * a0r, nav = -0.5 * dq.v;
-  const double upcc = q.u + cc, umcc = q.u - cc;
-  double spmR = umcc * k.dtdx + 1.0, sppR = upcc * k.dtdx + 1.0, sp0R = q.u * k.dtdx + 1.0;
-  double spmL = umcc * k.dtdx - 1.0, sppL = upcc * k.dtdx - 1.0, sp0L = q.u * k.dtdx - 1.0;
+// trace.c:85-147 in the fast and mixed forms: the four quotients from 1/rho
+// (ir) and 1/c (ic), and the predictor's products regrouped: -half * sp *
+// (half * x * rOcc) is sp * (x * (-rho / 4c)), and each face's sums fold into
+// multiply-adds.
+template <typename R>
+__device__ __forceinline__ void trace_fast(const Q4<R>& q, const Q4<R>& dq, R cc, R ic, R ir,
+                                           const TraceK<R>& k, Q4<R>& qxm, Q4<R>& qxp) {
+  const R one = (R) 1;
+  const R csq = cc * cc;
+  const R dpic = dq.p * ic;
+  const R dprcc = dpic * ir;              // dp / (rho c)
+  const R a0r = dq.r - dpic * ic;         // drho - dp / c^2
+  const R OrOcc = cc * ir;                // c / rho
+  const R qr4 = (R) -0.25 * (q.r * ic);   // -rho / (4c)
+  const R nap = (dprcc + dq.u) * qr4;     // -ap / 2
+  const R nam = (dprcc - dq.u) * qr4;     // -am / 2
+  const R na0 = (R) -0.5 * a0r, nav = (R) -0.5 * dq.v;
+  const R upcc = q.u + cc, umcc = q.u - cc;
+  R spmR = umcc * k.dtdx + one, sppR = upcc * k.dtdx + one, sp0R = q.u * k.dtdx + one;
+  R spmL = umcc * k.dtdx - one, sppL = upcc * k.dtdx - one, sp0L = q.u * k.dtdx - one;
   if (!wave_all(upcc < k.zeror && umcc > k.zerol)) {  // as in trace_tail
     spmR = (umcc >= k.zeror) ? k.project : spmR;
     sppR = (upcc >= k.zeror) ? k.project : sppR;
@@ -783,16 +802,16 @@ __device__ __forceinline__ void trace_fast(const Q4<double>& q, const Q4<double>
     sp0L = (q.u <= k.zerol) ? -k.project : sp0L;
   }
   {
-    const double amr = spmR * nam;
-    const double t = sppR * nap + amr;  // apr + amr
+    const R amr = spmR * nam;
+    const R t = sppR * nap + amr;  // apr + amr
     qxp.r = q.r + (t + sp0R * na0);
     qxp.u = q.u + (sppR * nap - amr) * OrOcc;
     qxp.v = q.v + sp0R * nav;
     qxp.p = q.p + t * csq;
   }
   {
-    const double aml = spmL * nam;
-    const double t = sppL * nap + aml;
+    const R aml = spmL * nam;
+    const R t = sppL * nap + aml;
     qxm.r = q.r + (t + sp0L * na0);
     qxm.u = q.u + (sppL * nap - aml) * OrOcc;
     qxm.v = q.v + sp0L * nav;
