@@ -55,6 +55,14 @@ FORM_NAMES = {0: "FP64 (bit for bit)", 1: "mixed FP32/FP64", 2: "FP64 fast form 
 PREC_ARG = {v: k for k, v in PRECISIONS.items()}
 
 
+_T0 = time.perf_counter()
+
+
+def progress(what):
+    """One progress line on stderr (the JSON result line goes to stdout)."""
+    print(f"bench.py [{time.perf_counter() - _T0:7.1f} s] {what}", file=sys.stderr, flush=True)
+
+
 def gpu_bench(nx, ny, steps, warmup, precision, autotune=True, testcase=0, prof_steps=20):
     """Times `steps` steps of one grid on the product path (graph replay), then
     an instrumented pass of prof_steps steps for per-kernel durations."""
@@ -81,6 +89,8 @@ def gpu_bench(nx, ny, steps, warmup, precision, autotune=True, testcase=0, prof_
     g.close()
     elapsed = t1 - t0
     cells = nx * ny
+    progress(f"{nx}x{ny} {FORM_NAMES[precision]}: {cells * steps / elapsed / 1e9:.2f} G "
+             "cell-updates/s")
     kern = max(("x_sweep", "y_sweep"), key=lambda k: prof[k][0])
     ms, n = prof[kern]
     bpc = BYTES_PER_CELL_SWEEP[precision]
@@ -145,6 +155,7 @@ def batched_bench(n=512, steps=100):
     del grids
     out["plain_fp64_per_grid"] = {"seconds": t_warm, "solves_per_s": n / t_warm,
                                   "cold_seconds": t_cold, "cold_solves_per_s": n / t_cold}
+    progress(f"batched: plain per-grid FP64 loop {t_warm:.2f} s warm, {t_cold:.2f} s cold")
     for name, prec in (("batched_fp64", hydro_amd.FP64), ("batched_fp64_fast", hydro_amd.FP64_FAST),
                        ("batched_mixed", hydro_amd.MIXED)):
         runs = W.batch_instances(n, steps=steps, precision=prec)
@@ -164,6 +175,7 @@ def batched_bench(n=512, steps=100):
                      out["cell_updates"] / t, "tune": tune, "setup_seconds": t0 - ts,
                      "speedup_vs_plain_fp64": t_warm / t,
                      "speedup_vs_plain_fp64_cold": t_cold / t}
+        progress(f"batched: {name} {t:.3f} s")
     return out
 
 
@@ -220,6 +232,7 @@ def serving_bench(n=256, rates=(20.0, 1000.0), chunk=20):
                                         "p99_s": float(np.percentile(lat, 99)),
                                         "max_s": float(lat.max()),
                                         "fp64_reruns": srv.reruns}})
+        progress(f"serving: {n} requests at {rate:g}/s in {span:.2f} s")
         if i == 0:
             keep_srv, keep_ids = srv, ids
             res["cell_updates"] = float(sum(r.params.nx * r.params.ny * x[5]
@@ -250,6 +263,8 @@ def serving_bench(n=256, rates=(20.0, 1000.0), chunk=20):
             e.append((float(d.sum() / np.abs(b).sum()), float(d.max() / np.abs(b).max())))
         errs.append(e)
         del um, u64
+        if len(warm) % 64 == 0:
+            progress(f"serving: plain FP64 loop, {len(warm)} of {n} runs")
     keep_srv.close()
     for s in served:
         runs = s.pop("runs")
@@ -593,12 +608,16 @@ def main():
         va = measured_valu(a.nx, prec, r["tune"], kern, r["kernels"][kern]["avg_ms"] / 1e3)
         if va:
             line["valu"] = va
+        progress("rocprofv3 traffic and VALU passes done")
     if not a.no_batched:
         line["batched"] = batched_bench(a.batch_n)
+        progress("batched workload done")
     if not a.no_serving:
         line["serving"] = serving_bench(a.serve_n, (a.serve_rate, 1000.0))
+        progress("serving workload done")
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.nx, ny, a.cpu_steps, a.testcase)
+        progress("CPU baseline done")
     print(json.dumps(line), flush=True)
 
 
