@@ -323,13 +323,19 @@ __device__ __forceinline__ Prim<R> prim_cell(R U0, R U1, R U2, R U3, const Phys<
     p = maxm(p, r * ph.smallp);
     o.c = P::sqrt1(P::qn(ph.gamma * p, rr));  // gamma*p > 0
   } else {
-    u = xdiv(U1, r);
-    v = xdiv(U2, r);
+    // FP32 (mixed form): one reciprocal of rho for the quotients, and c and
+    // 1/c (the trace's) from one v_rsq_f32 estimate
+    const R ir = __builtin_amdgcn_rcpf(r);
+    u = U1 * ir;
+    v = U2 * ir;
     R eken = half * (u * u + v * v);
-    e = div_sub(U3, r, eken);
+    e = __builtin_fmaf(U3, ir, -eken);
     p = ph.gm1 * r * e;
     p = maxm(p, r * ph.smallp);
-    o.c = xsqrt(xdiv(ph.gamma * p, r));
+    const R x = ph.gamma * p * ir;  // > 0
+    o.cinv = __builtin_amdgcn_rsqf(x);
+    o.c = x * o.cinv;
+    o.rinv = ir;
   }
   o.q.r = r;
   o.q.u = u;
@@ -990,13 +996,17 @@ __device__ __forceinline__ Q4<R> riemann_flux_mixed(const Q4<R>& ql, const Q4<R>
     if (!(fabs(d) > 1e-6 * (pst + ph.smallpp_d))) break;
   }
 
-  // Star state and sampling at x/t = 0 (riemann.c:211-276).
-  // (g6/pl, g6/pr and 1/wl, 1/wr shared with the Newton iteration and rstar)
+  // Star state and sampling at x/t = 0 (riemann.c:211-276), with the fast
+  // form's shared quotients: w and 1/w from one v_rsq_f32 estimate; with
+  // rstar = ro / X, one reciprocal of ro X gives 1/ro, 1/X and 1/rstar (1/smallr
+  // where the floor holds); sqrt(gamma po / ro) is sqrt(gamma po ro) / ro,
+  // the numerator the first guess's wl0 or wr0.
+  // (g6/pl, g6/pr shared with the Newton iteration)
   R ps = (R) pst;
-  R wr, wl;
-  xsqrt2(cr * __builtin_fmaf(ps - pr, (R) gpr, one), cl * __builtin_fmaf(ps - pl, (R) gpl, one),
-         wr, wl);
-  const R iwl = __builtin_amdgcn_rcpf(wl), iwr = __builtin_amdgcn_rcpf(wr);
+  const R xr = cr * __builtin_fmaf(ps - pr, (R) gpr, one);
+  const R xl = cl * __builtin_fmaf(ps - pl, (R) gpl, one);
+  const R iwr = __builtin_amdgcn_rsqf(xr), iwl = __builtin_amdgcn_rsqf(xl);
+  const R wr = xr * iwr, wl = xl * iwl;
   // ((ul + (pl-ps)/wl) + ur) - (pr-ps)/wr, the reference's left-to-right sum
   R ustar = half * __builtin_fmaf(-(pr - ps), iwr, __builtin_fmaf(pl - ps, iwl, ul) + ur);
   bool left = ustar > (R) 0;
@@ -1006,15 +1016,18 @@ __device__ __forceinline__ Q4<R> riemann_flux_mixed(const Q4<R>& ql, const Q4<R>
   R po = left ? pl : pr;
   R wo = left ? wl : wr;
   R iwo = left ? iwl : iwr;
-  R rstar = xdiv(ro, __builtin_fmaf(ro * (po - ps), iwo * iwo, one));
-  rstar = fmax(rstar, ph.smallr);
-  R co, cstar;  // (riemann.c computes co before rstar; neither depends on the other)
-  xsqrt2(fabs(xdiv(ph.gamma * po, ro)), fabs(xdiv(ph.gamma * ps, rstar)), co, cstar);
-  co = fmax(ph.smallc, co);
-  cstar = fmax(ph.smallc, cstar);
+  const R wo0 = left ? wl0 : wr0;
+  const R X = __builtin_fmaf(ro * (po - ps), iwo * iwo, one);
+  const R y = __builtin_amdgcn_rcpf(ro * X);
+  const R iro = X * y;
+  const R rs0 = ro * (ro * y);  // ro / X
+  const R rstar = fmax(rs0, ph.smallr);
+  const R irstar = rs0 > ph.smallr ? X * iro : (R) ph.smallr_inv;
+  R co = fmax(ph.smallc, wo0 * iro);
+  R cstar = fmax(ph.smallc, xsqrt(fabs(ph.gamma * ps * irstar)));
   R spout = co - sg * uo;
   R spin = cstar - sg * ustar;
-  R ushock = xdiv(wo, ro) - sg * uo;
+  R ushock = wo * iro - sg * uo;
   if (ps >= po) {
     spin = ushock;
     spout = ushock;

@@ -167,7 +167,8 @@ __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
     dq = slope_form<S, ST1, FD>(qm, pc.q, qp, ph);
   }
   Q4<S> qxm, qxp;
-  if constexpr (FD == kFast) trace_fast(pc.q, dq, pc.c, pc.cinv, pc.rinv, tk, qxm, qxp);
+  if constexpr (FD == kFast || sizeof(S) == 4)
+    trace_fast(pc.q, dq, pc.c, pc.cinv, pc.rinv, tk, qxm, qxp);
   else if (fc) trace_cell<S, true, true>(pc.q, dq, pc.c, tk, qxm, qxp, pc.rinv);
   else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);  // (fc is false in FP32)
   const Q4<S> qr = from_right4(qxp);
@@ -274,7 +275,7 @@ __device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
     dq = slope_form<S, ST1, FD>(lds_get<W>(sh.q, t), pc.q, lds_get<W>(sh.q, t + 2), ph);
   }
   Q4<S> qxm, qxp;
-  if constexpr (FD == kFast) {
+  if constexpr (FD == kFast || sizeof(S) == 4) {
     trace_fast(pc.q, dq, pc.c, pc.cinv, pc.rinv, tk, qxm, qxp);
   } else if constexpr (FD == kGated) {
     if (ft) trace_cell<S, true, true>(pc.q, dq, pc.c, tk, qxm, qxp, pc.rinv);  // ft implies fc
@@ -388,7 +389,8 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
                : prim_cell<S, false>(u.u0, u.u1, u.u2, u.u3, ph);
   };
   auto trace = [&](const Prim<S>& pq, const Q4<S>& dq, bool f, Q4<S>& qxm, Q4<S>& qxp) {
-    if constexpr (FD == kFast) trace_fast(pq.q, dq, pq.c, pq.cinv, pq.rinv, tk, qxm, qxp);
+    if constexpr (FD == kFast || sizeof(S) == 4)
+      trace_fast(pq.q, dq, pq.c, pq.cinv, pq.rinv, tk, qxm, qxp);
     else if (f) trace_cell<S, true>(pq.q, dq, pq.c, tk, qxm, qxp);
     else trace_cell<S, false>(pq.q, dq, pq.c, tk, qxm, qxp);
   };
@@ -436,7 +438,7 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
     Q4<S> qxmD, qxpD;
     {
       const Q4<S> dqD = o1 ? z : slope_form<S, ST1, FD>(qC, qD, pE.q, ph);
-      if constexpr (FD == kFast) trace_fast(qD, dqD, cD, iD, rD, tk, qxmD, qxpD);
+      if constexpr (FD == kFast || sizeof(S) == 4) trace_fast(qD, dqD, cD, iD, rD, tk, qxmD, qxpD);
       else if (fC && fD && fE) trace_cell<S, true, true>(qD, dqD, cD, tk, qxmD, qxpD, rD);
       else trace_cell<S, false>(qD, dqD, cD, tk, qxmD, qxpD);
     }
