@@ -9,6 +9,11 @@ in the FP64 fast form; the 23000x23000 states are checked for conservation
 of mass and total energy (reflective walls: conserved to round-off).  The
 100-step 8192x8192 comparison runs when HYDRO_LONG=1 (the oracle needs about
 a minute for it).
+
+At the benchmark's full lengths (200 steps of 8192x8192, 50 of 23000x23000)
+the oracle would need many minutes, so there the fast form is held to the
+contract against the FP64 form on the GPU, which is the oracle bit for bit
+on these grids (above) and on every trajectory test.
 """
 import math
 import os
@@ -109,3 +114,22 @@ def test_23000_corner_blast_bitwise_fast_and_conserved():
     u, dts = run_gpu(n, 1, 2, hydro_amd.FP64_FAST)
     check_fast(u, dts, ur, rdts, "23000^2, 2 steps")
     conserved(u, n)
+
+
+def test_8192_fast_form_200_steps_against_fp64_form():
+    """The 8192x8192 centred blast over the benchmark's 200 steps."""
+    u64, d64 = run_gpu(8192, 0, 200, hydro_amd.FP64)
+    u, dts = run_gpu(8192, 0, 200, hydro_amd.FP64_FAST)
+    check_fast(u, dts, u64, d64, "8192^2, 200 steps, against the FP64 form")
+
+
+@pytest.mark.skipif(os.environ.get("HYDRO_SKIP_23000") == "1", reason="asked to skip")
+def test_23000_fast_form_50_steps_against_fp64_form():
+    """The 23000x23000 corner blast over the benchmark's 50 steps (two
+    16.9 GB downloads); both forms conserve mass and total energy."""
+    n = 23000
+    u64, d64 = run_gpu(n, 1, 50, hydro_amd.FP64)
+    conserved(u64, n)
+    u, dts = run_gpu(n, 1, 50, hydro_amd.FP64_FAST)
+    conserved(u, n)
+    check_fast(u, dts, u64, d64, "23000^2, 50 steps, against the FP64 form")
