@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 }
 
 // Workgroup form (W = 2 or 4 waves): a workgroup walks xiter consecutive
-// row pieces of 64*W - 4 updated cells, neighbours through LDS.
+// row pieces of 64*W - 4 updated cells, neighbours through LDS (x_blocks).
 template <typename S, bool ST1, bool COUR, int W, int FD>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) void k_sweep_xb(GridGeom g, Phys<S> ph, double dx,
                                                      double dt_val, const RunCtl* ctl,
@@ -60,11 +60,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   const double dtdx = dt / dx;  // hydro_godunov.c:101
   const TraceK<S> tk = make_tracek<S>((S) dtdx, ph.scheme);
   double cx = 0.0, cy = 0.0;
-  SegWalk w((int) blockIdx.x * xiter, nseg);
-  for (int it = 0; it < xiter && w.row < g.ny; it++) {  // workgroup-uniform
-    x_block<S, ST1, COUR, W, FD>(g, ph, tk, dtdx, in, out, w.row, w.seg, sh, cx, cy);
-    w.advance(1, nseg);
-  }
+  x_blocks<S, ST1, COUR, W, FD>(g, ph, tk, dtdx, in, out, (int) blockIdx.x * xiter, xiter, nseg,
+                                sh, cx, cy);
   if (COUR) commit_courant(cx, cy, cour, kCourSlots, blockIdx.x % kCourSlots);
 }
 

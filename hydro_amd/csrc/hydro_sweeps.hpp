@@ -227,88 +227,146 @@ __device__ __forceinline__ Q4<S> lds_get(const S (&b)[4][64 * W + 2], int slot) 
   return Q4<S>{b[0][slot], b[1][slot], b[2][slot], b[3][slot]};
 }
 
-// The 64*W threads of a workgroup own 64*W consecutive cells of row jrow
-// (padded columns (64*W - 4)*seg + t) and update the middle 64*W - 4: the
-// same arithmetic as x_segment, with the neighbour values passed through LDS
-// (no VALU lane moves) and the 4-cell halo paid once per workgroup instead of
-// once per wave.  Every thread of the workgroup calls it with the same
-// (jrow, seg): it has barriers.  Consecutive calls may reuse `sh`: each
-// buffer is rewritten only after a barrier that follows its last read.
-template <typename S, bool ST1, bool COUR, int W, int FD>
-__device__ __forceinline__ void x_block(const GridGeom& g, const Phys<S>& ph,
-                                        const TraceK<S>& tk, double dtdx,
-                                        const GridView<S>& in, const GridView<S>& out,
-                                        long jrow, int seg, XLds<S, W>& sh, double& cx,
-                                        double& cy) {
-  constexpr int N = 64 * W;
-  const int t = threadIdx.x;
-  const long j = jrow + 2;
-  const int i = (N - 4) * seg + t;  // padded column of this thread
-  const int ic = i < g.nx + 4 ? i : g.nx + 3;
-  const long o = j * g.pitch + ic;
-
-  const S U0 = in.p[0][o], U1 = in.p[1][o], U2 = in.p[2][o], U3 = in.p[3][o];
-  // kGated: FastDiv for the primitives when the wave's cells are in range,
-  // for the traces when every wave's are (the slopes read neighbours of
-  // other waves)
+// One thread's cell of a row piece once loaded: its place, conserved values
+// and primitives (`fc`: kGated, the wave's cells passed cell_in).
+template <typename S>
+struct XCell {
+  long o;
+  int i;
+  S U0, U1, U2, U3;
   Prim<S> pc;
-  bool fc = false;
-  if constexpr (FD == kFast) {
-    pc = prim_fast(U0, U1, U2, U3, ph);
-  } else {
-    fc = FD == kGated && wave_cell_in<S>(U0, U1, U2, U3, ph);
-    pc = fc ? prim_cell<S, true>(U0, U1, U2, U3, ph) : prim_cell<S, false>(U0, U1, U2, U3, ph);
-  }
-  Q4<S> dq = {(S) 0, (S) 0, (S) 0, (S) 0};
-  bool ft = fc;
-  if (ph.iorder != 1) {  // launch-uniform
-    lds_put<W>(sh.q, t, pc.q);
-    if constexpr (FD == kGated) {
-      if ((t & 63) == 0) sh.in[t >> 6] = fc;
-    }
-    __syncthreads();
-    if constexpr (FD == kGated) {
-      int all = 1;
-      for (int w = 0; w < W; w++) all &= sh.in[w];
-      ft = __builtin_amdgcn_readfirstlane(all) != 0;  // the same in every lane
-    }
-    dq = slope_form<S, ST1, FD>(lds_get<W>(sh.q, t), pc.q, lds_get<W>(sh.q, t + 2), ph);
-  }
-  Q4<S> qxm, qxp;
-  if constexpr (FD == kFast || sizeof(S) == 4) {
-    trace_fast(pc.q, dq, pc.c, pc.cinv, pc.rinv, tk, qxm, qxp);
-  } else if constexpr (FD == kGated) {
-    if (ft) trace_cell<S, true, true>(pc.q, dq, pc.c, tk, qxm, qxp, pc.rinv);  // ft implies fc
-    else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);
-  } else {
-    trace_cell<S>(pc.q, dq, pc.c, tk, qxm, qxp);
-  }
-  lds_put<W>(sh.x, t, qxp);
-  __syncthreads();
-  const Q4<S> qr = lds_get<W>(sh.x, t + 2);
-  // flux through the face between this thread's cell and the next one
-  const Q4<double> fr = to_d4(riemann_flux<S, FD>(qxm, qr, ph));
-  lds_put<W>(sh.f, t, fr);
-  __syncthreads();
-  const Q4<double> fl = lds_get<W>(sh.f, t);
+  bool fc;
+};
 
-  const bool wr = t >= 2 && t <= N - 3 && i <= g.nx + 1;
-  const double n0 = update1<S>(U0, fl.r, fr.r, dtdx);
-  const double n1 = update1<S>(U1, fl.u, fr.u, dtdx);
-  const double n2 = update1<S>(U2, fl.v, fr.v, dtdx);
-  const double n3 = update1<S>(U3, fl.p, fr.p, dtdx);
-  if (wr) {
-    out.p[0][o] = (S) n0;
-    out.p[1][o] = (S) n1;
-    out.p[2][o] = (S) n2;
-    out.p[3][o] = (S) n3;
-    if (COUR) {
-      S ax, ay;
-      courant_cell<S, FD>((S) n0, (S) n1, (S) n2, (S) n3, ph, ax, ay);
-      cx = maxm(cx, (double) ax);
-      cy = maxm(cy, (double) ay);
+// The workgroup's 64*W threads own 64*W consecutive cells of a row (padded
+// columns (64*W - 4)*seg + t) and update the middle 64*W - 4: the same
+// arithmetic as x_segment, with the neighbour values passed through LDS (no
+// VALU lane moves) and the 4-cell halo paid once per workgroup instead of once
+// per wave.  The workgroup walks `npieces` consecutive row pieces from piece
+// `first`, software-pipelined: a piece's fluxes and the next piece's
+// primitives cross at one barrier (A), where the piece's update then runs
+// beside the next piece's slopes and traces, and the right-face states at
+// another (B): two barriers per piece.  Every LDS buffer is rewritten only
+// after a barrier that follows its last read: q (and the gated form's
+// per-wave flags `in`) for piece k+1 after B of piece k, read between A and B
+// of piece k+1; x before B, read between B and the next A; f between B and
+// the next A, read between that A and the next B.  Every thread of the
+// workgroup calls it with the same arguments: it has barriers.
+template <typename S, bool ST1, bool COUR, int W, int FD>
+__device__ __forceinline__ void x_blocks(const GridGeom& g, const Phys<S>& ph,
+                                         const TraceK<S>& tk, double dtdx,
+                                         const GridView<S>& in, const GridView<S>& out,
+                                         int first, int npieces, int nseg, XLds<S, W>& sh,
+                                         double& cx, double& cy) {
+  constexpr int N = 64 * W;
+  // the FP64 form's range-gated code has no registers to spare for it
+  constexpr bool kPrefetch = FD == kFast || sizeof(S) == 4;
+  const int t = threadIdx.x;
+  SegWalk w(first, nseg);
+  if (npieces <= 0 || w.row >= g.ny) return;  // workgroup-uniform
+  // a piece's cell: its loads, then its primitives (published)
+  auto fetch = [&](const SegWalk& ww) {
+    XCell<S> c;
+    c.i = (N - 4) * ww.seg + t;
+    const int ic = c.i < g.nx + 4 ? c.i : g.nx + 3;
+    c.o = ((long) ww.row + 2) * g.pitch + ic;
+    c.U0 = in.p[0][c.o];
+    c.U1 = in.p[1][c.o];
+    c.U2 = in.p[2][c.o];
+    c.U3 = in.p[3][c.o];
+    return c;
+  };
+  auto prims = [&](XCell<S>& c) {
+    c.fc = false;
+    // kGated: FastDiv for the primitives when the wave's cells are in range,
+    // for the traces when every wave's are (the slopes read neighbours of
+    // other waves)
+    if constexpr (FD == kFast) {
+      c.pc = prim_fast(c.U0, c.U1, c.U2, c.U3, ph);
+    } else {
+      c.fc = FD == kGated && wave_cell_in<S>(c.U0, c.U1, c.U2, c.U3, ph);
+      c.pc = c.fc ? prim_cell<S, true>(c.U0, c.U1, c.U2, c.U3, ph)
+                  : prim_cell<S, false>(c.U0, c.U1, c.U2, c.U3, ph);
     }
+    if (ph.iorder != 1) {  // launch-uniform
+      lds_put<W>(sh.q, t, c.pc.q);
+      if constexpr (FD == kGated) {
+        if ((t & 63) == 0) sh.in[t >> 6] = c.fc;
+      }
+    }
+  };
+  // update a piece from its own right-face flux and the left neighbour's
+  auto update = [&](const XCell<S>& c, const Q4<double>& fr) {
+    const Q4<double> fl = lds_get<W>(sh.f, t);
+    const bool wr = t >= 2 && t <= N - 3 && c.i <= g.nx + 1;
+    const double n0 = update1<S>(c.U0, fl.r, fr.r, dtdx);
+    const double n1 = update1<S>(c.U1, fl.u, fr.u, dtdx);
+    const double n2 = update1<S>(c.U2, fl.v, fr.v, dtdx);
+    const double n3 = update1<S>(c.U3, fl.p, fr.p, dtdx);
+    if (wr) {
+      out.p[0][c.o] = (S) n0;
+      out.p[1][c.o] = (S) n1;
+      out.p[2][c.o] = (S) n2;
+      out.p[3][c.o] = (S) n3;
+      if (COUR) {
+        S ax, ay;
+        courant_cell<S, FD>((S) n0, (S) n1, (S) n2, (S) n3, ph, ax, ay);
+        cx = maxm(cx, (double) ax);
+        cy = maxm(cy, (double) ay);
+      }
+    }
+  };
+  XCell<S> cur = fetch(w), prev, next;
+  prims(cur);
+  Q4<double> fprev;
+  bool have_prev = false;
+  for (int k = 0;;) {
+    __syncthreads();  // A: cur's primitives, prev's fluxes
+    if (have_prev) update(prev, fprev);
+    Q4<S> dq = {(S) 0, (S) 0, (S) 0, (S) 0};
+    bool ft = cur.fc;
+    if (ph.iorder != 1) {
+      if constexpr (FD == kGated) {
+        int all = 1;
+        for (int v = 0; v < W; v++) all &= sh.in[v];
+        ft = __builtin_amdgcn_readfirstlane(all) != 0;  // the same in every lane
+      }
+      dq = slope_form<S, ST1, FD>(lds_get<W>(sh.q, t), cur.pc.q, lds_get<W>(sh.q, t + 2), ph);
+    }
+    Q4<S> qxm, qxp;
+    if constexpr (FD == kFast || sizeof(S) == 4) {
+      trace_fast(cur.pc.q, dq, cur.pc.c, cur.pc.cinv, cur.pc.rinv, tk, qxm, qxp);
+    } else if constexpr (FD == kGated) {
+      if (ft) trace_cell<S, true, true>(cur.pc.q, dq, cur.pc.c, tk, qxm, qxp, cur.pc.rinv);
+      else trace_cell<S, false>(cur.pc.q, dq, cur.pc.c, tk, qxm, qxp);  // (ft implies fc)
+    } else {
+      trace_cell<S>(cur.pc.q, dq, cur.pc.c, tk, qxm, qxp);
+    }
+    lds_put<W>(sh.x, t, qxp);
+    __syncthreads();  // B: the right-face states
+    // (fast and mixed forms) the next piece's loads go out before this
+    // piece's Riemann solves
+    SegWalk wn = w;
+    wn.advance(1, nseg);
+    const bool more = k + 1 < npieces && wn.row < g.ny;  // workgroup-uniform
+    if constexpr (kPrefetch) {
+      if (more) next = fetch(wn);
+    }
+    const Q4<S> qr = lds_get<W>(sh.x, t + 2);
+    // flux through the face between this thread's cell and the next one
+    fprev = to_d4(riemann_flux<S, FD>(qxm, qr, ph));
+    lds_put<W>(sh.f, t, fprev);
+    prev = cur;
+    have_prev = true;
+    if (!more) break;
+    w = wn;
+    k++;
+    if constexpr (kPrefetch) cur = next;
+    else cur = fetch(w);
+    prims(cur);
   }
+  __syncthreads();  // A: the last piece's fluxes
+  update(prev, fprev);
 }
 
 // ---- y sweep: one column, rows [j0, j1) ----------------------------------------
