@@ -46,8 +46,11 @@ __device__ __forceinline__ int next_task(uint64_t* qnext) {
 // limiter's multiply/divide by slope_type is compiled out as in single grids.
 // FD: the arithmetic form (persistent workgroups keep the SIMDs busy, so the
 // FP64 batches take the range-gated divisions, kGated, for every instance).
+// Built for 5 waves per SIMD, as the single-grid x sweep (the per-task state
+// would otherwise take it to 98 VGPRs and 4 waves; one register spills).
 template <typename S, bool ST1, int FD>
-__global__ __launch_bounds__(256) void k_batch_sweep_x(const InstDev<S>* __restrict__ inst,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_batch_sweep_x(
+    const InstDev<S>* __restrict__ inst,
                                                        const BTask* __restrict__ tasks,
                                                        int ntask, int phase, int xiter,
                                                        uint64_t* qnext) {
