@@ -50,10 +50,8 @@ __device__ __forceinline__ int next_task(uint64_t* qnext) {
 // would otherwise take it to 98 VGPRs and 4 waves; one register spills).
 template <typename S, bool ST1, int FD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_batch_sweep_x(
-    const InstDev<S>* __restrict__ inst,
-                                                       const BTask* __restrict__ tasks,
-                                                       int ntask, int phase, int xiter,
-                                                       uint64_t* qnext) {
+    const InstDev<S>* __restrict__ inst, const BTask* __restrict__ tasks, int ntask, int phase,
+    int xiter, uint64_t* qnext) {
   const int wpb = blockDim.x >> 6;
   const int wave = wave_id();
   for (;;) {
