@@ -8,6 +8,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -744,6 +745,82 @@ static double time_sweep(hydro_amd_grid* g, int idim, const SweepTune& t, int re
   return ms / reps;
 }
 
+// Indices of the k smallest scores.
+static std::vector<int> best_of(const std::vector<double>& ms, int k) {
+  std::vector<int> idx(ms.size());
+  for (size_t c = 0; c < ms.size(); c++) idx[c] = (int) c;
+  std::sort(idx.begin(), idx.end(), [&](int a, int b) { return ms[a] < ms[b]; });
+  if ((int) idx.size() > k) idx.resize(k);
+  return idx;
+}
+
+// Whole steps with each candidate shape, from a saved copy of the state and
+// the CFL slots that is put back before each trial and at the end, with t,
+// nstep and dt, so the run continues exactly as if untuned: the isolated
+// launches of the first pass miss how a shape fares between the other kernels
+// of a step (tails, caches), which moved their pick by a few per cent.
+// Returns the fastest candidate, or `fallback` when the copy cannot be
+// allocated or a call fails.
+static SweepTune tune_by_steps(hydro_amd_grid* g, const std::vector<SweepTune>& cand,
+                               const SweepTune& fallback) {
+  if (cand.size() < 2 || g->prof) return fallback;
+  const size_t bytes = (size_t) (4 * g->plane + kOrigin + 64) * g->esz;
+  const size_t slot_bytes = 2 * kCourSlots * sizeof(uint64_t);
+  char* save = nullptr;
+  if (hipMalloc(&save, bytes + slot_bytes) != hipSuccess) {
+    (void) hipGetLastError();
+    return fallback;
+  }
+  const double t0 = g->t, dt0 = g->dt;
+  const int n0 = g->nstep;
+  const bool slots_valid = g->cour_ver == g->state_ver;
+  const bool dirty0 = g->cour_dirty;
+  bool ok = hipMemcpyAsync(save, g->mem[0], bytes, hipMemcpyDeviceToDevice, g->st) ==
+                hipSuccess &&
+            hipMemcpyAsync(save + bytes, g->cour, slot_bytes, hipMemcpyDeviceToDevice, g->st) ==
+                hipSuccess;
+  auto restore = [&] {
+    ok = ok &&
+         hipMemcpyAsync(g->mem[0], save, bytes, hipMemcpyDeviceToDevice, g->st) == hipSuccess &&
+         hipMemcpyAsync(g->cour, save + bytes, slot_bytes, hipMemcpyDeviceToDevice, g->st) ==
+             hipSuccess;
+    g->t = t0;
+    g->nstep = n0;
+    g->dt = dt0;
+    bump(g);
+    g->cour_ver = slots_valid ? g->state_ver : 0;
+    g->cour_dirty = dirty0;
+    ok = ok && write_ctl(g) == 0;
+  };
+  constexpr int kSteps = 4;  // two of each parity (the odd ones fold the CFL maxima)
+  std::vector<double> ms(cand.size(), 0.0);
+  hipEvent_t a = nullptr, b = nullptr;
+  ok = ok && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess;
+  for (int pass = 0; pass < 2 && ok; pass++)
+    for (size_t k = 0; k < cand.size() && ok; k++) {
+      const size_t c = pass ? cand.size() - 1 - k : k;
+      g->tune = cand[c];
+      drop_graph(g);
+      restore();  // a first run builds the shape's graph
+      ok = ok && run_enqueue(g, 2, 0) == 0 && run_finish(g, nullptr) == 0;
+      restore();
+      if (n0 % 2 == 0) ensure_cour(g);  // (outside the clock)
+      ok = ok && hipEventRecord(a, g->st) == hipSuccess && run_enqueue(g, kSteps, 0) == 0 &&
+           hipEventRecord(b, g->st) == hipSuccess && run_finish(g, nullptr) == 0;
+      float e = 0;
+      ok = ok && hipEventElapsedTime(&e, a, b) == hipSuccess;
+      ms[c] += e;
+    }
+  if (a) (void) hipEventDestroy(a);
+  if (b) (void) hipEventDestroy(b);
+  restore();
+  (void) hipStreamSynchronize(g->st);
+  (void) hipFree(save);
+  drop_graph(g);
+  if (!ok) return fallback;
+  return cand[best_of(ms, 1)[0]];
+}
+
 namespace {
 struct TuneKey {
   int nx, ny, prec;
@@ -823,6 +900,17 @@ extern "C" int hydro_amd_grid_autotune(hydro_amd_grid* g, int reps, int* best) {
     if (xms[c] < bx) { bx = xms[c]; bt.xwaves = xs[c].xwaves; bt.xiter = xs[c].xiter; }
   (void) hipStreamSynchronize(g->st);
   (void) hipFree(scour);
+  HIPCHK(hipGetLastError());
+  // the three best of each direction, combined, timed as whole steps
+  std::vector<SweepTune> combos;
+  for (int yi : best_of(yms, 3))
+    for (int xi : best_of(xms, 3)) {
+      SweepTune t = ys[yi];
+      t.xwaves = xs[xi].xwaves;
+      t.xiter = xs[xi].xiter;
+      combos.push_back(t);
+    }
+  bt = tune_by_steps(g, combos, bt);
   HIPCHK(hipGetLastError());
   g->tune = bt;
   drop_graph(g);

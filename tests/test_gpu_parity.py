@@ -210,6 +210,40 @@ def test_autotune_keeps_state_and_results(port):
     assert_same(u, ur, f"after autotune {best}")
 
 
+def test_autotune_mid_run_keeps_the_run(port):
+    """Tuned between steps (after an odd step, with dt still to come from the
+    CFL maxima its last sweep folded): the autotuner's whole-step trials run
+    from a saved copy of the state and the CFL slots, so the run continues
+    bit for bit -- against the oracle in the FP64 form, against an untuned
+    run in the fast form."""
+    p = hydro_amd.Params(**{**CORNER_250x125, "nx": 300, "ny": 200, "dx": 1 / 300.0})
+    po = to_oracle(p, testcase=1)
+    ur = port.init(po)
+    rdt, _ = port.run(po, ur, 9)
+    with hydro_amd.Grid(p) as g:
+        g.init(1)
+        d1 = g.run(3)
+        state = g.state()
+        g.autotune(reps=1)
+        assert g.state() == state
+        d2 = g.run(6)
+        u = g.download()
+    assert np.array_equal(np.concatenate([d1, d2]), rdt)
+    assert_same(u, ur, "FP64, tuned after step 3")
+    pf = hydro_amd.Params(**{**p.__dict__, "precision": hydro_amd.FP64_FAST})
+    outs = []
+    for tune in (False, True):
+        with hydro_amd.Grid(pf) as g:
+            g.init(1)
+            d1 = g.run(3)
+            if tune:
+                g.autotune(reps=1)
+            d2 = g.run(6)
+            outs.append((np.concatenate([d1, d2]), g.download()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert bits_differ(outs[0][1], outs[1][1]) == 0
+
+
 def test_mid_size_against_oracle(port):
     """1024x768 centred blast, 20 steps, bitwise."""
     p = hydro_amd.Params(**{**CORNER_250x125, "nx": 1024, "ny": 768, "dx": 1 / 1024.0})
