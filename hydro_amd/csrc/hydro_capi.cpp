@@ -550,7 +550,10 @@ static int get_graph(hydro_amd_grid* g) {
   return 0;
 }
 
-static int run_enqueue(hydro_amd_grid* g, int max_steps, int log_cap) {
+// `start`, when given, is recorded on the stream right before the first step
+// (the autotuner's clock, which the run control's set-up stays outside of).
+static int run_enqueue(hydro_amd_grid* g, int max_steps, int log_cap,
+                       hipEvent_t start = nullptr) {
   if (max_steps <= 0) return 0;
   // dt of an even step comes from the CFL maxima of the current state
   if (g->nstep % 2 == 0) ensure_cour(g);
@@ -566,6 +569,7 @@ static int run_enqueue(hydro_amd_grid* g, int max_steps, int log_cap) {
   // rewrites it before the next synchronising call
   *g->hctl = c;
   HIPCHK(hipMemcpyAsync(g->ctl, g->hctl, sizeof(RunCtl), hipMemcpyHostToDevice, g->st));
+  if (start) HIPCHK(hipEventRecord(start, g->st));
   int n = g->nstep, left = max_steps;
   if (n % 2 == 1) {  // finish the odd step first so the graph starts even
     enq_step(g, n);
@@ -792,7 +796,7 @@ static SweepTune tune_by_steps(hydro_amd_grid* g, const std::vector<SweepTune>& 
     g->cour_dirty = dirty0;
     ok = ok && write_ctl(g) == 0;
   };
-  constexpr int kSteps = 4;  // two of each parity (the odd ones fold the CFL maxima)
+  constexpr int kSteps = 8;  // of each parity, half (the odd ones fold the CFL maxima)
   std::vector<double> ms(cand.size(), 0.0);
   hipEvent_t a = nullptr, b = nullptr;
   ok = ok && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess;
@@ -805,8 +809,8 @@ static SweepTune tune_by_steps(hydro_amd_grid* g, const std::vector<SweepTune>& 
       ok = ok && run_enqueue(g, 2, 0) == 0 && run_finish(g, nullptr) == 0;
       restore();
       if (n0 % 2 == 0) ensure_cour(g);  // (outside the clock)
-      ok = ok && hipEventRecord(a, g->st) == hipSuccess && run_enqueue(g, kSteps, 0) == 0 &&
-           hipEventRecord(b, g->st) == hipSuccess && run_finish(g, nullptr) == 0;
+      ok = ok && run_enqueue(g, kSteps, 0, a) == 0 && hipEventRecord(b, g->st) == hipSuccess &&
+           run_finish(g, nullptr) == 0;
       float e = 0;
       ok = ok && hipEventElapsedTime(&e, a, b) == hipSuccess;
       ms[c] += e;
