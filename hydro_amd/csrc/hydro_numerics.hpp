@@ -603,34 +603,24 @@ __device__ __forceinline__ void trace_tail(const Q4<R>& q, const Q4<R>& dq, R cc
   }
 }
 
-template <typename R>
-__device__ __forceinline__ void trace_fast(const Q4<R>& q, const Q4<R>& dq, R cc, R ic, R ir,
-                                           const TraceK<R>& k, Q4<R>& qxm, Q4<R>& qxp);
-
-// trace.c:85-147 for one cell (quotients as the form computes them, then
-// trace_tail).  FAST (FP64): FastDiv, for cells whose own and neighbour
-// values (the slope inputs) passed cell_in.  RINV: rinv is prim_cell's
-// reciprocal of q.r, reused for cc / q.r.
+// trace.c:85-147 for one cell in the FP64 form (quotients as the form
+// computes them, then trace_tail; the fast and mixed forms take trace_fast).
+// FAST: FastDiv, for cells whose own and neighbour values (the slope inputs)
+// passed cell_in.  RINV: rinv is prim_cell's reciprocal of q.r, reused for
+// cc / q.r.
 template <typename R, bool FAST = false, bool RINV = false>
 __device__ __forceinline__ void trace_cell(const Q4<R>& q, const Q4<R>& dq, R cc,
                                            const TraceK<R>& k, Q4<R>& qxm, Q4<R>& qxp,
                                            R rinv = (R) 0) {
-  R csq = cc * cc;
-  R rOcc, OrOcc, dprcc, a0r;
-  if constexpr (sizeof(R) == 8) {
-    using P = std::conditional_t<FAST, FastDiv, IeeeDiv>;
-    rOcc = P::divn(q.r, cc);  // rho, c > 0
-    if constexpr (FAST && RINV) OrOcc = qdiv_nz(cc, Rcp{q.r, rinv});
-    else OrOcc = P::divn(cc, q.r);
-    dprcc = P::div(dq.p, q.r * cc);
-    a0r = dq.r - P::div(dq.p, csq);
-  } else {
-    // FP32 (mixed form): two reciprocals for the four quotients, then the
-    // fast form's regrouped predictor
-    trace_fast<R>(q, dq, cc, __builtin_amdgcn_rcpf(cc), __builtin_amdgcn_rcpf(q.r), k, qxm,
-                  qxp);
-    return;
-  }
+  static_assert(sizeof(R) == 8, "the FP32 form takes trace_fast");
+  const R csq = cc * cc;
+  using P = std::conditional_t<FAST, FastDiv, IeeeDiv>;
+  const R rOcc = P::divn(q.r, cc);  // rho, c > 0
+  R OrOcc;
+  if constexpr (FAST && RINV) OrOcc = qdiv_nz(cc, Rcp{q.r, rinv});
+  else OrOcc = P::divn(cc, q.r);
+  const R dprcc = P::div(dq.p, q.r * cc);
+  const R a0r = dq.r - P::div(dq.p, csq);
   trace_tail<R>(q, dq, cc, csq, rOcc, OrOcc, dprcc, a0r, k, qxm, qxp);
 }
 

@@ -170,7 +170,7 @@ __device__ __forceinline__ void x_segment(const GridGeom& g, const Phys<S>& ph,
   if constexpr (FD == kFast || sizeof(S) == 4)
     trace_fast(pc.q, dq, pc.c, pc.cinv, pc.rinv, tk, qxm, qxp);
   else if (fc) trace_cell<S, true, true>(pc.q, dq, pc.c, tk, qxm, qxp, pc.rinv);
-  else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);  // (fc is false in FP32)
+  else trace_cell<S, false>(pc.q, dq, pc.c, tk, qxm, qxp);
   const Q4<S> qr = from_right4(qxp);
   // flux through the face between this lane's cell and the next one
   const Q4<double> fr = to_d4(riemann_flux<S, FD>(qxm, qr, ph));
