@@ -250,16 +250,11 @@ __device__ __forceinline__ bool cell_in(S U0, S U1, S U2, S U3, const Phys<S>& p
 // correctly rounded multi-instruction sequences.
 __device__ __forceinline__ double xdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float xdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-// c + a/b and a/b - c: in FP64 exactly as written (the quotient rounded,
-// then the sum); in FP32 the quotient a * rcp(b) fuses into the sum (one
-// v_fma_f32).
+// c + a/b: in FP64 exactly as written (the quotient rounded, then the sum);
+// in FP32 the quotient a * rcp(b) fuses into the sum (one v_fma_f32).
 __device__ __forceinline__ double add_div(double c, double a, double b) { return c + a / b; }
 __device__ __forceinline__ float add_div(float c, float a, float b) {
   return __builtin_fmaf(a, __builtin_amdgcn_rcpf(b), c);
-}
-__device__ __forceinline__ double div_sub(double a, double b, double c) { return a / b - c; }
-__device__ __forceinline__ float div_sub(float a, float b, float c) {
-  return __builtin_fmaf(a, __builtin_amdgcn_rcpf(b), -c);
 }
 __device__ __forceinline__ double xsqrt(double a) { return sqrt_fast(a); }
 __device__ __forceinline__ float xsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
@@ -562,8 +557,8 @@ __device__ __forceinline__ void trace_tail(const Q4<R>& q, const Q4<R>& dq, R cc
   R a0v = dq.v;
   R upcc = q.u + cc;
   R umcc = q.u - cc;
-  // The predictor factors (s * dtdx +- one: one multiply-add where the unit
-  // contracts, the reference's product then sum in the FP64 unit), clamped to
+  // The predictor factors (s * dtdx +- one: the reference's product then sum,
+  // the FP64 unit contracting nothing), clamped to
   // +-project where s has crossed zeror / zerol.  With umcc <= u <= upcc no
   // clamp applies on a lane with upcc < zeror and umcc > zerol, so the
   // selects run only when some lane of the wave fails that test (every lane
