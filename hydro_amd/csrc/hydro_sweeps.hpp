@@ -25,6 +25,20 @@
 
 namespace hydro {
 
+// ---- grid stores of the y sweep ------------------------------------------------
+
+// The y march writes each cell of its column strip once, in 64-column row
+// pieces, and a whole state is far larger than the caches: its stores carry
+// the non-temporal hint (global_store ... nt), so the written lines stream out
+// instead of displacing the rows the march is still to read (measured on the
+// 8192x8192 blast: y sweep 3 % faster in the FP64 fast form; the x sweeps,
+// whose row pieces overlap by their halos, and the loads are faster without
+// the hint).  The values are unchanged.
+template <typename S>
+__device__ __forceinline__ void st_stream(S* p, S v) {
+  __builtin_nontemporal_store(v, p);
+}
+
 // ---- lane exchange -----------------------------------------------------------
 
 // lane l receives the value of lane l-1 (DPP wave_ror:1, so lane 0 receives
@@ -515,10 +529,10 @@ __device__ __forceinline__ void y_march(const GridGeom& g, const Phys<S>& ph,
     const double n3 = update1<S>(uC.u3, fprev.p, fcur.p, dtdx);
     if (colok) {
       // scatter back with the swap undone (conservar.c:177-180)
-      out.p[0][oc] = (S) n0;
-      out.p[2][oc] = (S) n1;
-      out.p[1][oc] = (S) n2;
-      out.p[3][oc] = (S) n3;
+      st_stream(&out.p[0][oc], (S) n0);
+      st_stream(&out.p[2][oc], (S) n1);
+      st_stream(&out.p[1][oc], (S) n2);
+      st_stream(&out.p[3][oc], (S) n3);
       if (COUR) {
         S ax, ay;
         courant_cell<S, FD>((S) n0, (S) n2, (S) n1, (S) n3, ph, ax, ay);
