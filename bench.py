@@ -21,8 +21,10 @@ events recorded on the grid's own stream around every launch).  `roofline` is
 for the dominant sweep kernel: algorithmic bytes (64 B per cell per sweep in
 FP64: read 4 doubles, write 4) over its average duration; its `traffic` is
 the HBM bytes the kernel really moves (two rocprofv3 --pmc passes,
-calibrated on a copy kernel).  `valu` is that kernel's VALU work from a
-third counter pass.  `c2` is the 23000x23000 corner blast (50 steps).
+calibrated on a copy kernel), and its `copy_ceiling` the rate of a plain copy
+in the same access shape (the sweep's bytes with no arithmetic: what this
+pattern reaches on the machine, beside the published peak).  `valu` is that
+kernel's VALU work from a third counter pass.  `c2` is the 23000x23000 corner blast (50 steps).
 `batched` and `serving` time the many-grid workloads (512 batched instances;
 256 Poisson arrivals at a nominal and a saturating rate, mixed precision)
 against the plain per-grid FP64 loop, timed cold (each grid created, run and
@@ -580,6 +582,16 @@ def main():
         "kernels": r["kernels"],
         "kernels_note": r["kernels_note"],
     }
+    # the bandwidth this access shape reaches with no arithmetic, beside the
+    # published peak
+    import hydro_amd
+    ceil = hydro_amd.stream_probe((a.nx + 4) * (ny + 4), 10)
+    line["roofline"]["copy_ceiling"] = {
+        "GBps": round(ceil, 1), "frac": round(line["roofline"]["achieved"] / ceil, 4),
+        "kernel": "k_stream_probe: a copy in the sweep's access shape (4 arrays of "
+                  f"{(a.nx + 4) * (ny + 4)} doubles in, 4 out, one cell per thread, "
+                  "non-temporal loads and stores), 64 bytes per cell"}
+    progress(f"streaming-copy ceiling {ceil:.0f} GB/s")
     if not a.no_forms:
         forms = {a.precision: form_summary(r, prec)}
         for name, p2 in sorted(PRECISIONS.items()):

@@ -182,6 +182,8 @@ def lib() -> C.CDLL:
     L.hydro_amd_now.restype = C.c_double
     L.hydro_amd_copy_probe.argtypes = [C.c_longlong, C.c_int]
     L.hydro_amd_copy_probe.restype = C.c_double
+    L.hydro_amd_stream_probe.argtypes = [C.c_longlong, C.c_int]
+    L.hydro_amd_stream_probe.restype = C.c_double
     L.hydro_amd_server_create.argtypes = [C.c_int, C.c_longlong, C.c_int, C.c_int]
     L.hydro_amd_server_create.restype = S
     L.hydro_amd_server_destroy.argtypes = [S]
@@ -441,6 +443,15 @@ class Batch:
 def copy_probe(nbytes: int, reps: int = 10) -> float:
     """GB/s (read + written) of a plain copy kernel over nbytes."""
     v = lib().hydro_amd_copy_probe(int(nbytes), reps)
+    if v < 0:
+        raise HydroError(last_error())
+    return float(v)
+
+
+def stream_probe(cells: int, reps: int = 10) -> float:
+    """GB/s (64 bytes per cell) of a copy in the sweeps' access shape: four
+    arrays of `cells` doubles in, four out, non-temporal loads and stores."""
+    v = lib().hydro_amd_stream_probe(int(cells), reps)
     if v < 0:
         raise HydroError(last_error())
     return float(v)

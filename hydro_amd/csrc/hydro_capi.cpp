@@ -972,6 +972,41 @@ extern "C" double hydro_amd_copy_probe(long long bytes, int reps) {
   return 2.0 * n * 8 * reps / (ms * 1e-3) / 1e9;
 }
 
+extern "C" double hydro_amd_stream_probe(long long cells, int reps) {
+  // (at most 2^36 cells: far beyond any device memory, and 2^28 workgroups)
+  if (cells < 1 || cells > (1LL << 36) || reps < 1) return fail("bad stream probe size");
+  const long n = (long) cells;
+  StreamProbe4 p{};
+  double* bufs[8] = {};
+  for (int k = 0; k < 8; k++) {
+    if (hipMalloc(&bufs[k], n * 8) != hipSuccess) {
+      for (int j = 0; j < k; j++) (void) hipFree(bufs[j]);
+      return fail("stream probe allocation failed");
+    }
+    if (k < 4) (void) hipMemset(bufs[k], 0, n * 8);
+  }
+  for (int k = 0; k < 4; k++) {
+    p.in[k] = bufs[k];
+    p.out[k] = bufs[4 + k];
+  }
+  hipEvent_t e0, e1;
+  (void) hipEventCreate(&e0);
+  (void) hipEventCreate(&e1);
+  launch_stream_probe(p, n, nullptr);  // warm
+  launch_stream_probe(p, n, nullptr);
+  (void) hipEventRecord(e0, nullptr);
+  for (int r = 0; r < reps; r++) launch_stream_probe(p, n, nullptr);
+  (void) hipEventRecord(e1, nullptr);
+  (void) hipEventSynchronize(e1);
+  float ms = 0;
+  (void) hipEventElapsedTime(&ms, e0, e1);
+  (void) hipEventDestroy(e0);
+  (void) hipEventDestroy(e1);
+  for (int k = 0; k < 8; k++) (void) hipFree(bufs[k]);
+  // bytes moved per launch: 4 arrays of n doubles read + 4 written
+  return 64.0 * n * reps / (ms * 1e-3) / 1e9;
+}
+
 // ---- reference call surface ------------------------------------------------------------
 
 // Globals the reference's driver expects (main.c:69, perfcnt.c); weak so a

@@ -99,6 +99,11 @@ void launch_init(const GridGeom& g, GridView<S> a, GridView<S> b, int bx, int by
                  double e_blast, hipStream_t st);
 
 void launch_copy_probe(const double* a, double* b, long n, hipStream_t st);
+struct StreamProbe4 {
+  const double* in[4];
+  double* out[4];
+};
+void launch_stream_probe(const StreamProbe4& p, long n, hipStream_t st);
 
 // Density image (hydro_image.hip): clears *key, reduces the maximum density
 // of the padded plane `rho` into it, then writes w*h RGB pixels (3 bytes

@@ -219,6 +219,13 @@ int hydro_amd_server_reserve_results(hydro_amd_server *s, long long bytes);
  * and to calibrate HBM byte counters. */
 double hydro_amd_copy_probe(long long bytes, int reps);
 
+/* Measurement support (no counterpart in the reference): a copy in the
+ * sweeps' access shape -- four arrays of `cells` doubles read, four written,
+ * one cell per thread, non-temporal loads and stores (k_stream_probe) -- run
+ * `reps` times; returns GB/s counting 64 bytes per cell, the bandwidth a
+ * sweep could reach with no arithmetic.  Allocates 64 * cells bytes. */
+double hydro_amd_stream_probe(long long cells, int reps);
+
 /* Allocation-free compute_deltat.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  Same result as
